@@ -1,0 +1,89 @@
+"""ctypes binding of libfsamd.so (include/fake_spectra_amd.h).
+
+The library is required: every entry point of this package raises if it is
+missing or no GPU is visible.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libfsamd.so")
+
+FSAMD_OK = 0
+FSAMD_ERR_ARG = 1
+FSAMD_ERR_OOM = 2
+FSAMD_ERR_HIP = 3
+FSAMD_ERR_UNSUPPORTED = 4
+MODE_BATCHED = 0
+MODE_PLAIN = 1
+
+# Every symbol include/fake_spectra_amd.h declares.
+EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan_destroy",
+           "fsamd_plan_npairs", "fsamd_plan_pairs", "fsamd_plan_near_lines", "fsamd_tau",
+           "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats")
+
+
+class FsamdLine(C.Structure):
+    _fields_ = [("lambda_cm", C.c_double), ("gamma", C.c_double), ("fosc", C.c_double),
+                ("amumass", C.c_double), ("dens", C.c_void_p), ("out", C.c_void_p)]
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+def _declare(L):
+    vp, d, i, ll = C.c_void_p, C.c_double, C.c_int, C.c_longlong
+    L.fsamd_last_error.argtypes = []
+    L.fsamd_last_error.restype = C.c_char_p
+    L.fsamd_version.argtypes = []
+    L.fsamd_version.restype = i
+    L.fsamd_plan_create.argtypes = [d, vp, vp, ll, vp, vp, i, vp, C.POINTER(vp)]
+    L.fsamd_plan_create.restype = i
+    L.fsamd_plan_destroy.argtypes = [vp]
+    L.fsamd_plan_destroy.restype = i
+    L.fsamd_plan_npairs.argtypes = [vp]
+    L.fsamd_plan_npairs.restype = ll
+    L.fsamd_plan_pairs.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+    L.fsamd_plan_pairs.restype = i
+    L.fsamd_plan_near_lines.argtypes = [vp, vp, C.POINTER(ll), vp]
+    L.fsamd_plan_near_lines.restype = i
+    L.fsamd_tau.argtypes = [vp, i, i, d, d, vp, vp, vp, vp, C.POINTER(FsamdLine), i, i, vp]
+    L.fsamd_tau.restype = i
+    L.fsamd_colden.argtypes = [vp, i, i, d, vp, vp, C.POINTER(FsamdLine), i, vp]
+    L.fsamd_colden.restype = i
+    L.fsamd_particle_interpolate.argtypes = [i, i, i, d, d, d, d, d, d, d, d, vp, vp, vp, vp, vp, ll, vp, vp,
+                                             i, vp, vp]
+    L.fsamd_particle_interpolate.restype = i
+    L.fsamd_set_stats.argtypes = [i]
+    L.fsamd_set_stats.restype = i
+    L.fsamd_last_stats.argtypes = [C.POINTER(ll)]
+    L.fsamd_last_stats.restype = i
+
+
+def lib():
+    """Load libfsamd.so once; raises RuntimeError if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError("libfsamd.so is not built (%s); run python -m fake_spectra_amd.build" % LIB_PATH)
+            L = C.CDLL(LIB_PATH)
+            _declare(L)
+            _lib = L
+    return _lib
+
+
+def check(rc: int):
+    """Map a libfsamd status to the reference's exception classes."""
+    if rc == FSAMD_OK:
+        return
+    msg = lib().fsamd_last_error().decode(errors="replace")
+    if rc == FSAMD_ERR_ARG or rc == FSAMD_ERR_UNSUPPORTED:
+        raise ValueError(msg)
+    if rc == FSAMD_ERR_OOM:
+        raise MemoryError(msg)
+    raise RuntimeError("libfsamd error %d: %s" % (rc, msg))

@@ -1,0 +1,720 @@
+// Sightline <-> particle binning on the GPU.
+//
+// The reference (IndexTable, fake_spectra/index_table.cpp:7-150) sorts
+// sightlines into two std::multimaps keyed by one transverse coordinate,
+// range-queries them per particle, and inserts every accepted
+// (sightline, particle, dr2) into per-sightline std::maps under an OpenMP
+// critical section.  Here:
+//   1. sightlines are bucketed into one square grid per axis over their two
+//      transverse coordinates (a superset search structure only);
+//   2. one thread per particle visits the grid cells its support touches and
+//      applies the reference's acceptance test exactly (float key window with
+//      periodic wrap, float second-coordinate window, FP64 minimum-image dr2
+//      compared with the float product h*h), counting then emitting pairs
+//      into a particle-major list at offsets from an exclusive scan;
+//   3. a stable LSD radix sort on the sightline id (LDS digit histograms,
+//      LDS prefix scans, wave-ballot ranking) turns it into per-sightline
+//      rows in ascending particle order, the std::map order of the reference.
+// All arithmetic that decides membership is compiled without FP contraction.
+#include "fsamd_internal.h"
+#include "plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+namespace fsamd {
+
+// ---------------------------------------------------------------------------
+// Small utilities
+
+__global__ void fill_i32_kernel(int *p, int v, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void fill_f64_kernel(double *p, double v, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static int grid_for(long long n, int threads, int cap = 65536)
+{
+    long long b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+int launch_fill_i32(int *p, int value, long long n, hipStream_t s)
+{
+    if (n <= 0) return FSAMD_OK;
+    fill_i32_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, value, n);
+    FSAMD_HIP(hipGetLastError());
+    return FSAMD_OK;
+}
+
+int launch_fill_f64(double *p, double value, long long n, hipStream_t s)
+{
+    if (n <= 0) return FSAMD_OK;
+    fill_f64_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, value, n);
+    FSAMD_HIP(hipGetLastError());
+    return FSAMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan (tile reduce -> scan of tile sums -> apply).
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ inline long long wave_incl_scan(long long v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        long long t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over the 256 threads of a block; *total gets the block sum.
+__device__ inline long long block_excl_scan(long long v, long long *lds4, long long *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long inc = wave_incl_scan(v);
+    if (lane == 63) lds4[w] = inc;
+    __syncthreads();
+    long long off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kScanThreads / 64; ++i) {
+        if (i < w) off += lds4[i];
+        tot += lds4[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+template <typename T>
+__global__ void scan_reduce_kernel(const T *in, long long n, long long *sums)
+{
+    __shared__ long long lds4[kScanThreads / 64];
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    long long s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+        if (base + i < n) s += (long long)in[base + i];
+    long long tot;
+    block_excl_scan(s, lds4, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// out[i] = tile_off[tile] + exclusive prefix within the tile; out may alias in
+template <typename T>
+__global__ void scan_apply_kernel(const T *in, long long *out, long long n, const long long *tile_off)
+{
+    __shared__ long long lds4[kScanThreads / 64];
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    long long s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? (long long)in[base + i] : 0;
+        s += v[i];
+    }
+    long long tot;
+    long long off = block_excl_scan(s, lds4, &tot) + (tile_off ? tile_off[blockIdx.x] : 0);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = off;
+        off += v[i];
+    }
+}
+
+size_t exclusive_scan_ws_bytes(long long n)
+{
+    // every level holds ceil(n/tile)+1 tile sums and recurses on that many
+    size_t total = 0;
+    n += 1;
+    while (n > kScanTile) {
+        long long nt = (n + kScanTile - 1) / kScanTile + 1;
+        total += sizeof(long long) * (size_t)nt;
+        n = nt;
+    }
+    return total + 256;
+}
+
+// In-place exclusive scan of int64 data[0..n); ws from exclusive_scan_ws_bytes.
+static int scan_i64_rec(long long *data, long long n, char *ws, size_t ws_bytes, hipStream_t s)
+{
+    if (n <= 0) return FSAMD_OK;
+    long long nt = (n + kScanTile - 1) / kScanTile;
+    if (nt == 1) {
+        scan_apply_kernel<long long><<<1, kScanThreads, 0, s>>>(data, data, n, nullptr);
+        FSAMD_HIP(hipGetLastError());
+        return FSAMD_OK;
+    }
+    size_t need = sizeof(long long) * (size_t)(nt + 1);
+    if (need > ws_bytes) return fail(FSAMD_ERR_ARG, "scan workspace too small");
+    long long *sums = (long long *)ws;
+    scan_reduce_kernel<long long><<<(int)nt, kScanThreads, 0, s>>>(data, n, sums);
+    FSAMD_HIP(hipGetLastError());
+    int rc = scan_i64_rec(sums, nt, ws + need, ws_bytes - need, s);
+    if (rc) return rc;
+    scan_apply_kernel<long long><<<(int)nt, kScanThreads, 0, s>>>(data, data, n, sums);
+    FSAMD_HIP(hipGetLastError());
+    return FSAMD_OK;
+}
+
+int exclusive_scan_i64_inplace(long long *data, long long n, void *ws, size_t ws_bytes, hipStream_t s)
+{
+    return scan_i64_rec(data, n, (char *)ws, ws_bytes, s);
+}
+
+// out[0..n] = exclusive scan of int32 in[0..n), out[n] = total.
+int exclusive_scan_i32_to_i64(const int *in, long long *out, long long n, void *ws, size_t ws_bytes,
+                              hipStream_t s)
+{
+    long long nt = (n + kScanTile - 1) / kScanTile;
+    if (n <= 0) {
+        FSAMD_HIP(hipMemsetAsync(out, 0, sizeof(long long), s));
+        return FSAMD_OK;
+    }
+    size_t need = sizeof(long long) * (size_t)(nt + 1);
+    if (need > ws_bytes) return fail(FSAMD_ERR_ARG, "scan workspace too small");
+    long long *sums = (long long *)ws;
+    scan_reduce_kernel<int><<<(int)nt, kScanThreads, 0, s>>>(in, n, sums);
+    FSAMD_HIP(hipGetLastError());
+    // sums[nt] := 0 so that after the scan sums[nt] holds the grand total
+    FSAMD_HIP(hipMemsetAsync(sums + nt, 0, sizeof(long long), s));
+    int rc = scan_i64_rec(sums, nt + 1, (char *)ws + need, ws_bytes - need, s);
+    if (rc) return rc;
+    scan_apply_kernel<int><<<(int)nt, kScanThreads, 0, s>>>(in, out, n, sums);
+    FSAMD_HIP(hipGetLastError());
+    FSAMD_HIP(hipMemcpyAsync(out + n, sums + nt, sizeof(long long), hipMemcpyDeviceToDevice, s));
+    return FSAMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sightline grid
+
+// Plane coordinates of a sightline: (key, second) as compared by the reference.
+__device__ inline void los_plane(int ax, const double *c, double &k1, double &k2)
+{
+    if (ax == 1) {
+        k1 = c[1];
+        k2 = c[2];
+    } else if (ax == 2) {
+        k1 = c[0];
+        k2 = c[2];
+    } else {
+        k1 = c[0];
+        k2 = c[1];
+    }
+}
+
+__device__ inline int wrap_cell(double k, double box, double inv_cw, int nc)
+{
+    double kw = k - floor(k / box) * box;
+    int c = (int)floor(kw * inv_cw);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__global__ void los_axis_count_kernel(const int *axis, int nlos, int *counts3)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nlos) {
+        int a = axis[i];
+        if (a >= 1 && a <= 3) atomicAdd(&counts3[a - 1], 1);
+    }
+}
+
+__global__ void los_cell_kernel(const int *axis, const double *cofm, int nlos, double box, LosGrid g,
+                                int *cell_of, int *cell_count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlos) return;
+    int a = axis[i];
+    double k1, k2;
+    los_plane(a, &cofm[3 * (size_t)i], k1, k2);
+    // sightlines with non-finite coordinates never pass the reference's tests
+    if (a < 1 || a > 3 || !isfinite(k1) || !isfinite(k2)) {
+        cell_of[i] = -1;
+        return;
+    }
+    const int ai = a - 1;
+    const int c1 = wrap_cell(k1, box, g.inv_cw[ai], g.nc[ai]);
+    const int c2 = wrap_cell(k2, box, g.inv_cw[ai], g.nc[ai]);
+    const int cell = g.base[ai] + c2 * g.nc[ai] + c1;
+    cell_of[i] = cell;
+    atomicAdd(&cell_count[cell], 1);
+}
+
+__global__ void los_scatter_kernel(const int *axis, const double *cofm, int nlos, const int *cell_of,
+                                   int *cursor, int *cell_ids, double2 *cell_kl)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlos) return;
+    int cell = cell_of[i];
+    if (cell < 0) return;
+    int at = atomicAdd(&cursor[cell], 1);
+    double k1, k2;
+    los_plane(axis[i], &cofm[3 * (size_t)i], k1, k2);
+    cell_ids[at] = i;
+    cell_kl[at] = make_double2(k1, k2);
+}
+
+__global__ void i64_to_i32_kernel(const long long *in, int *out, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = (int)in[i];
+}
+
+// ---------------------------------------------------------------------------
+// The reference acceptance test (index_table.cpp:22-113), exact.
+//   first/second: the particle's float coordinates along the sightline's key
+//   and second axes; key/lproj2: the sightline's coordinates; hh: float h.
+__device__ inline bool los_accept(float first, float second, float hh, double key, double lproj2,
+                                  double box, double &dr2)
+{
+    // get_nearby: half-open key window [ffm, ffp) with periodic wrap (93-112)
+    float ffp = first + hh;
+    if ((double)ffp > box) ffp = (float)((double)ffp - box);
+    float ffm = first - hh;
+    if (ffm < 0) ffm = (float)((double)ffm + box);
+    const bool inwin = (ffm <= ffp) ? (key >= (double)ffm && key < (double)ffp)
+                                    : (key < (double)ffp || key >= (double)ffm);
+    if (!inwin) return false;
+    // second_close (52-68)
+    bool close = false;
+    float sp = second + hh;
+    if ((double)sp > box && lproj2 < (double)sp - box) close = true;
+    float sm = second - hh;
+    if (!close && sm < 0 && lproj2 > (double)sm + box) close = true;
+    if (!close && lproj2 > (double)sm && lproj2 < (double)sp) close = true;
+    if (!close) return false;
+    // calc_dr2 (70-87) and the acceptance dr2 <= hh*hh with a float product (45)
+    double dr = fabs((double)first - key);
+    if (dr > 0.5 * box) dr = box - dr;
+    double d2 = dr * dr;
+    dr = fabs((double)second - lproj2);
+    if (dr > 0.5 * box) dr = box - dr;
+    d2 += dr * dr;
+    dr2 = d2;
+    const float hh2 = hh * hh;
+    return d2 <= (double)hh2;
+}
+
+// Particle coordinates (first, second) for a sightline axis (index_table.cpp:33-40, 117-127).
+__device__ inline void particle_plane(int ax, const float *p, float &first, float &second)
+{
+    if (ax == 1) {
+        first = p[1];
+        second = p[2];
+    } else if (ax == 2) {
+        first = p[0];
+        second = p[2];
+    } else {
+        first = p[0];
+        second = p[1];
+    }
+}
+
+// Visit every sightline accepted for particle p; f(sightline, dr2).
+template <class F>
+__device__ inline void visit_accepted(const BinArgs &A, const float *p, float hh, F &&f)
+{
+    const double ah = fabs((double)hh);
+    if (!(ah < 1e300)) return;  // NaN / inf support: the reference's windows are empty or undefined
+    for (int ai = 0; ai < 3; ++ai) {
+        if (A.grid.nlos_axis[ai] == 0) continue;
+        const int ax = ai + 1;
+        float first, second;
+        particle_plane(ax, p, first, second);
+        const int nc = A.grid.nc[ai];
+        const double inv = A.grid.inv_cw[ai];
+        const double hr = ah * (1.0 + 1e-5) + 1e-3 * A.grid.cw[ai];
+        long long lo1 = (long long)floor(((double)first - hr) * inv);
+        long long hi1 = (long long)floor(((double)first + hr) * inv);
+        long long lo2 = (long long)floor(((double)second - hr) * inv);
+        long long hi2 = (long long)floor(((double)second + hr) * inv);
+        if (hi1 - lo1 + 1 >= nc) {
+            lo1 = 0;
+            hi1 = nc - 1;
+        }
+        if (hi2 - lo2 + 1 >= nc) {
+            lo2 = 0;
+            hi2 = nc - 1;
+        }
+        for (long long c2 = lo2; c2 <= hi2; ++c2) {
+            int w2 = (int)(((c2 % nc) + nc) % nc);
+            for (long long c1 = lo1; c1 <= hi1; ++c1) {
+                int w1 = (int)(((c1 % nc) + nc) % nc);
+                const int cell = A.grid.base[ai] + w2 * nc + w1;
+                const int e = A.grid.cell_start[cell + 1];
+                for (int k = A.grid.cell_start[cell]; k < e; ++k) {
+                    const double2 kl = A.grid.cell_kl[k];
+                    double dr2;
+                    if (los_accept(first, second, hh, kl.x, kl.y, A.box, dr2)) f(A.grid.cell_ids[k], dr2);
+                }
+            }
+        }
+    }
+}
+
+__global__ void bin_count_kernel(BinArgs A, int *cnt)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.npart;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float p[3] = {A.pos[3 * i], A.pos[3 * i + 1], A.pos[3 * i + 2]};
+        int c = 0;
+        visit_accepted(A, p, A.h[i], [&](int, double) { ++c; });
+        cnt[i] = c;
+    }
+}
+
+__global__ void bin_emit_kernel(BinArgs A, const long long *poff, int *keys, int *vals)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.npart;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float p[3] = {A.pos[3 * i], A.pos[3 * i + 1], A.pos[3 * i + 2]};
+        long long at = poff[i];
+        visit_accepted(A, p, A.h[i], [&](int s, double) {
+            keys[at] = s;
+            vals[at] = (int)i;
+            ++at;
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, value) int32 pairs by key digits.
+
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 16;
+constexpr long long kRsTile = (long long)kRsThreads * kRsItems;
+constexpr int kRsMaxBits = 8;
+constexpr int kRsWaves = kRsThreads / 64;
+
+__global__ void rs_upsweep_kernel(const int *keys, long long n, int shift, int bits, long long ntiles,
+                                  long long *table)
+{
+    __shared__ int hist[1 << kRsMaxBits];
+    const int nb = 1 << bits;
+    for (int d = threadIdx.x; d < nb; d += kRsThreads) hist[d] = 0;
+    __syncthreads();
+    const long long base = blockIdx.x * kRsTile;
+    const unsigned mask = (unsigned)nb - 1;
+    for (int r = 0; r < kRsItems; ++r) {
+        const long long e = base + r * kRsThreads + threadIdx.x;
+        if (e < n) atomicAdd(&hist[((unsigned)keys[e] >> shift) & mask], 1);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < nb; d += kRsThreads) table[(long long)d * ntiles + blockIdx.x] = hist[d];
+}
+
+__global__ void rs_downsweep_kernel(const int *kin, const int *vin, int *kout, int *vout, long long n,
+                                    int shift, int bits, long long ntiles, const long long *table)
+{
+    __shared__ long long base_off[1 << kRsMaxBits];
+    __shared__ int run[1 << kRsMaxBits];
+    __shared__ int wcnt[kRsWaves][1 << kRsMaxBits];
+    const int nb = 1 << bits;
+    const unsigned mask = (unsigned)nb - 1;
+    for (int d = threadIdx.x; d < nb; d += kRsThreads) {
+        base_off[d] = table[(long long)d * ntiles + blockIdx.x];
+        run[d] = 0;
+        for (int w = 0; w < kRsWaves; ++w) wcnt[w][d] = 0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const long long tbase = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        const long long e = tbase + r * kRsThreads + threadIdx.x;
+        const bool valid = e < n;
+        int k = 0, v = 0;
+        unsigned d = 0;
+        if (valid) {
+            k = kin[e];
+            v = vin[e];
+            d = ((unsigned)k >> shift) & mask;
+        }
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            long long off = base_off[d] + run[d] + rank;
+            for (int w = 0; w < wave; ++w) off += wcnt[w][d];
+            kout[off] = k;
+            vout[off] = v;
+        }
+        __syncthreads();
+        for (int dd = threadIdx.x; dd < nb; dd += kRsThreads) {
+            int s = 0;
+            for (int w = 0; w < kRsWaves; ++w) {
+                s += wcnt[w][dd];
+                wcnt[w][dd] = 0;
+            }
+            run[dd] += s;
+        }
+        __syncthreads();
+    }
+}
+
+// offsets[s] = first position of key >= s in the sorted keys (s = 0..nlos)
+__global__ void row_offsets_kernel(const int *keys, long long n, int nlos, long long *offsets)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nlos) return;
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        long long mid = (lo + hi) >> 1;
+        if (keys[mid] < s) lo = mid + 1;
+        else hi = mid;
+    }
+    offsets[s] = lo;
+}
+
+__global__ void near_flag_kernel(const long long *poff, long long n, int *flag)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        flag[i] = poff[i + 1] > poff[i] ? 1 : 0;
+}
+
+__global__ void near_scatter_kernel(const long long *poff, const long long *fscan, long long n, int *out)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        if (poff[i + 1] > poff[i]) out[fscan[i]] = (int)i;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int DevBuf::alloc(size_t n)
+{
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        (void)hipGetLastError();
+        return fail(FSAMD_ERR_OOM, "hipMalloc of " + std::to_string(n) + " bytes failed");
+    }
+    bytes = n;
+    return FSAMD_OK;
+}
+
+static int choose_cells(int n)
+{
+    if (n <= 0) return 1;
+    int c = (int)std::ceil(std::sqrt((double)n));
+    return std::max(1, std::min(c, 4096));
+}
+
+int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long long npart,
+               const int *axis, const double *cofm, int nlos, hipStream_t s)
+{
+    int rc;
+    P->box = box;
+    P->npart = npart;
+    P->nlos = nlos;
+    if ((rc = P->axis.alloc(sizeof(int) * (size_t)nlos))) return rc;
+    if ((rc = P->cofm.alloc(sizeof(double) * 3 * (size_t)nlos))) return rc;
+    if (nlos > 0) {
+        FSAMD_HIP(hipMemcpyAsync(P->axis.ptr, axis, sizeof(int) * nlos, hipMemcpyDeviceToDevice, s));
+        FSAMD_HIP(hipMemcpyAsync(P->cofm.ptr, cofm, sizeof(double) * 3 * nlos, hipMemcpyDeviceToDevice, s));
+    }
+    // --- sightline grid
+    DevBuf counts3;
+    if ((rc = counts3.alloc(sizeof(int) * 4))) return rc;
+    FSAMD_HIP(hipMemsetAsync(counts3.ptr, 0, sizeof(int) * 4, s));
+    if (nlos > 0) {
+        los_axis_count_kernel<<<(nlos + 255) / 256, 256, 0, s>>>(P->axis.as<int>(), nlos, counts3.as<int>());
+        FSAMD_HIP(hipGetLastError());
+    }
+    int hc[4] = {0, 0, 0, 0};
+    FSAMD_HIP(hipMemcpyAsync(hc, counts3.ptr, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    FSAMD_HIP(hipStreamSynchronize(s));
+    LosGrid &g = P->grid;
+    int ncells = 0;
+    for (int a = 0; a < 3; ++a) {
+        g.nlos_axis[a] = hc[a];
+        g.nc[a] = choose_cells(hc[a]);
+        g.cw[a] = box / g.nc[a];
+        g.inv_cw[a] = g.nc[a] / box;
+        g.base[a] = ncells;
+        ncells += g.nc[a] * g.nc[a];
+    }
+    P->ncells = ncells;
+    DevBuf cell_of, cell_count, cell_cursor, scan_ws, cell_start64;
+    if ((rc = cell_of.alloc(sizeof(int) * (size_t)std::max(nlos, 1)))) return rc;
+    if ((rc = cell_count.alloc(sizeof(int) * (size_t)(ncells + 1)))) return rc;
+    if ((rc = cell_start64.alloc(sizeof(long long) * (size_t)(ncells + 1)))) return rc;
+    if ((rc = P->cell_start.alloc(sizeof(int) * (size_t)(ncells + 1)))) return rc;
+    if ((rc = P->cell_ids.alloc(sizeof(int) * (size_t)std::max(nlos, 1)))) return rc;
+    if ((rc = P->cell_kl.alloc(sizeof(double2) * (size_t)std::max(nlos, 1)))) return rc;
+    if ((rc = scan_ws.alloc(exclusive_scan_ws_bytes(std::max<long long>(npart, ncells) + 1)))) return rc;
+    if ((rc = launch_fill_i32(cell_count.as<int>(), 0, ncells + 1, s))) return rc;
+    if (nlos > 0) {
+        los_cell_kernel<<<(nlos + 255) / 256, 256, 0, s>>>(P->axis.as<int>(), P->cofm.as<double>(), nlos, box, g,
+                                                           cell_of.as<int>(), cell_count.as<int>());
+        FSAMD_HIP(hipGetLastError());
+    }
+    if ((rc = exclusive_scan_i32_to_i64(cell_count.as<int>(), cell_start64.as<long long>(), ncells,
+                                        scan_ws.ptr, scan_ws.bytes, s)))
+        return rc;
+    i64_to_i32_kernel<<<grid_for(ncells + 1, 256), 256, 0, s>>>(cell_start64.as<long long>(),
+                                                                P->cell_start.as<int>(), ncells + 1);
+    FSAMD_HIP(hipGetLastError());
+    if ((rc = cell_cursor.alloc(sizeof(int) * (size_t)(ncells + 1)))) return rc;
+    FSAMD_HIP(hipMemcpyAsync(cell_cursor.ptr, P->cell_start.ptr, sizeof(int) * (ncells + 1),
+                             hipMemcpyDeviceToDevice, s));
+    if (nlos > 0) {
+        los_scatter_kernel<<<(nlos + 255) / 256, 256, 0, s>>>(P->axis.as<int>(), P->cofm.as<double>(), nlos,
+                                                              cell_of.as<int>(), cell_cursor.as<int>(),
+                                                              P->cell_ids.as<int>(), P->cell_kl.as<double2>());
+        FSAMD_HIP(hipGetLastError());
+    }
+    g.cell_start = P->cell_start.as<int>();
+    g.cell_ids = P->cell_ids.as<int>();
+    g.cell_kl = P->cell_kl.as<double2>();
+
+    // --- count, scan, emit
+    BinArgs A;
+    A.box = box;
+    A.pos = pos;
+    A.h = h;
+    A.npart = npart;
+    A.grid = g;
+    DevBuf cnt;
+    if ((rc = cnt.alloc(sizeof(int) * (size_t)std::max<long long>(npart, 1)))) return rc;
+    if ((rc = P->poff.alloc(sizeof(long long) * (size_t)(npart + 1)))) return rc;
+    if (npart > 0) {
+        bin_count_kernel<<<grid_for(npart, 256, 1 << 20), 256, 0, s>>>(A, cnt.as<int>());
+        FSAMD_HIP(hipGetLastError());
+    }
+    if ((rc = exclusive_scan_i32_to_i64(cnt.as<int>(), P->poff.as<long long>(), npart, scan_ws.ptr,
+                                        scan_ws.bytes, s)))
+        return rc;
+    long long M = 0;
+    FSAMD_HIP(hipMemcpyAsync(&M, P->poff.as<long long>() + npart, sizeof(long long), hipMemcpyDeviceToHost, s));
+    FSAMD_HIP(hipStreamSynchronize(s));
+    cnt.release();
+    P->npairs = M;
+    DevBuf k0, v0, k1, v1;
+    const size_t mbytes = sizeof(int) * (size_t)std::max<long long>(M, 1);
+    if ((rc = k0.alloc(mbytes)) || (rc = v0.alloc(mbytes)) || (rc = k1.alloc(mbytes)) || (rc = v1.alloc(mbytes)))
+        return rc;
+    if (npart > 0 && M > 0) {
+        bin_emit_kernel<<<grid_for(npart, 256, 1 << 20), 256, 0, s>>>(A, P->poff.as<long long>(), k0.as<int>(),
+                                                                      v0.as<int>());
+        FSAMD_HIP(hipGetLastError());
+    }
+    // --- stable radix sort by sightline id, ping-pong between (k0,v0) and (k1,v1)
+    int total_bits = 1;
+    while ((1ll << total_bits) < (long long)nlos) ++total_bits;
+    const int passes = (total_bits + kRsMaxBits - 1) / kRsMaxBits;
+    const int bits_per = (total_bits + passes - 1) / passes;
+    DevBuf *kin = &k0, *vin = &v0, *kout = &k1, *vout = &v1;
+    if (M > 0) {
+        const long long ntiles = (M + kRsTile - 1) / kRsTile;
+        DevBuf table, tws;
+        if ((rc = table.alloc(sizeof(long long) * (size_t)((1 << bits_per) * ntiles + 1)))) return rc;
+        if ((rc = tws.alloc(exclusive_scan_ws_bytes((1 << bits_per) * ntiles + 1)))) return rc;
+        for (int ps = 0; ps < passes; ++ps) {
+            const int shift = ps * bits_per;
+            const int bits = std::min(bits_per, total_bits - shift);
+            rs_upsweep_kernel<<<(int)ntiles, kRsThreads, 0, s>>>(kin->as<int>(), M, shift, bits, ntiles,
+                                                                 table.as<long long>());
+            FSAMD_HIP(hipGetLastError());
+            if ((rc = exclusive_scan_i64_inplace(table.as<long long>(), (1ll << bits) * ntiles, tws.ptr,
+                                                 tws.bytes, s)))
+                return rc;
+            rs_downsweep_kernel<<<(int)ntiles, kRsThreads, 0, s>>>(kin->as<int>(), vin->as<int>(), kout->as<int>(),
+                                                                   vout->as<int>(), M, shift, bits, ntiles,
+                                                                   table.as<long long>());
+            FSAMD_HIP(hipGetLastError());
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
+    }
+    // (kin, vin) hold the sorted sightline ids and particle ids
+    P->pair_part.take(*vin);
+    if ((rc = P->line_off.alloc(sizeof(long long) * (size_t)(nlos + 1)))) return rc;
+    row_offsets_kernel<<<(nlos + 1 + 255) / 256, 256, 0, s>>>(kin->as<int>(), M, nlos,
+                                                             P->line_off.as<long long>());
+    FSAMD_HIP(hipGetLastError());
+
+    // --- work order: sightlines by descending pair count (stable)
+    std::vector<long long> off(nlos + 1);
+    FSAMD_HIP(hipMemcpyAsync(off.data(), P->line_off.ptr, sizeof(long long) * (nlos + 1),
+                             hipMemcpyDeviceToHost, s));
+    FSAMD_HIP(hipStreamSynchronize(s));
+    std::vector<int> order(nlos);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return off[a + 1] - off[a] > off[b + 1] - off[b];
+    });
+    P->max_row = 0;
+    for (int i = 0; i < nlos; ++i) P->max_row = std::max(P->max_row, off[i + 1] - off[i]);
+    if ((rc = P->order.alloc(sizeof(int) * (size_t)std::max(nlos, 1)))) return rc;
+    if (nlos > 0)
+        FSAMD_HIP(hipMemcpy(P->order.ptr, order.data(), sizeof(int) * nlos, hipMemcpyHostToDevice));
+    P->order_host = std::move(order);
+    P->off_host = std::move(off);
+    return FSAMD_OK;
+}
+
+int plan_near_lines(const fsamd_plan *P, int *out, long long *n_out, hipStream_t s)
+{
+    int rc;
+    const long long n = P->npart;
+    if (n == 0) {
+        *n_out = 0;
+        return FSAMD_OK;
+    }
+    DevBuf flag, fscan, ws;
+    if ((rc = flag.alloc(sizeof(int) * (size_t)n))) return rc;
+    if ((rc = fscan.alloc(sizeof(long long) * (size_t)(n + 1)))) return rc;
+    if ((rc = ws.alloc(exclusive_scan_ws_bytes(n + 1)))) return rc;
+    near_flag_kernel<<<grid_for(n, 256), 256, 0, s>>>(P->poff.as<long long>(), n, flag.as<int>());
+    FSAMD_HIP(hipGetLastError());
+    if ((rc = exclusive_scan_i32_to_i64(flag.as<int>(), fscan.as<long long>(), n, ws.ptr, ws.bytes, s)))
+        return rc;
+    near_scatter_kernel<<<grid_for(n, 256), 256, 0, s>>>(P->poff.as<long long>(), fscan.as<long long>(), n, out);
+    FSAMD_HIP(hipGetLastError());
+    long long cnt = 0;
+    FSAMD_HIP(hipMemcpyAsync(&cnt, fscan.as<long long>() + n, sizeof(long long), hipMemcpyDeviceToHost, s));
+    FSAMD_HIP(hipStreamSynchronize(s));
+    *n_out = cnt;
+    return FSAMD_OK;
+}
+
+}  // namespace fsamd

@@ -1,0 +1,24 @@
+// Work items and launchers of the deposit kernels.
+#pragma once
+#include "fsamd_internal.h"
+
+namespace fsamd {
+
+// Pairs [p0, p1) of sightline `los` for line `line`.  `direct` items own the
+// whole sightline and store the row; others add theirs with FP64 atomics
+// into a zero-filled output.
+struct WorkItem {
+    int los;
+    int line;
+    long long p0;
+    long long p1;
+    int direct;
+    int pad;
+};
+
+size_t deposit_lds_bytes(int nbins);
+int launch_deposit(bool tau, bool lds, const DepositArgs &A, const WorkItem *items, int item_base, int nblocks,
+                   int use_queue, hipStream_t s);
+int deposit_blocks_per_cu(bool tau, bool lds, int nbins);
+
+}  // namespace fsamd

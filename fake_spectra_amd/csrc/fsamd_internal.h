@@ -1,0 +1,99 @@
+// Internal declarations shared by the HIP translation units of libfsamd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "../../include/fake_spectra_amd.h"
+
+namespace fsamd {
+
+// Physical constants (fake_spectra/absorption.cpp:21-33, absorption.h:4).
+constexpr double kSigmaT = 6.652458558e-25;
+constexpr double kBoltzmann = 1.3806504e-16;
+constexpr double kLight = 2.99792458e10;
+constexpr double kProtonMass = 1.67262178e-24;
+constexpr double kPi = 3.14159265358979323846;
+
+constexpr int kTopHat = 0;       // singleabs.h:9
+constexpr int kCubicSpline = 1;  // singleabs.h:10
+constexpr int kNGrid = 8;        // singleabs.h:8
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define FSAMD_HIP(call)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::fsamd::fail(FSAMD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ----------------------------------------------------------------------------
+// Sightline grid: sightlines bucketed by their transverse coordinates, one
+// square grid per axis.  The plane coordinates of a sightline are the two
+// quantities the reference's IndexTable compares against (index_table.cpp:7-50):
+//   axis 1: key = cofm.y, second = cofm.z ; axis 2: key = cofm.x, second = cofm.z
+//   axis 3: key = cofm.x, second = cofm.y
+
+struct LosGrid {
+    int nc[3];             // cells per side
+    int base[3];           // first cell id of each axis grid
+    int nlos_axis[3];
+    double cw[3];          // cell width
+    double inv_cw[3];
+    const int *cell_start;     // [ncells + 1]
+    const int *cell_ids;       // sightline ids in cell order
+    const double2 *cell_kl;    // (key, second) of each entry
+};
+
+struct BinArgs {
+    double box;
+    const float *pos;      // [N,3]
+    const float *h;        // [N]
+    long long npart;
+    LosGrid grid;
+};
+
+// Deposit arguments shared by the tau and colden kernels.
+struct LineDesc {
+    double sigma_a;    // absorption.cpp:154
+    double bfac;       // absorption.cpp:155
+    double voigt_fac;  // absorption.cpp:156
+    const float *dens; // [N] species density
+    double *out;       // [nlos, nbins]
+};
+
+constexpr int kMaxLines = 8;   // lines per deposit launch
+
+struct DepositArgs {
+    int nbins;
+    int kernel;
+    double velfac;
+    double vbox;       // box * velfac
+    double box;
+    double tautail;
+    const float *pos;   // [N,3]
+    const float *vel;   // [N,3]
+    const float *temp;  // [N]
+    const float *h;     // [N]
+    const int *axis;    // [L]
+    const double *cofm; // [L,3]
+    const long long *line_off;  // [L+1]
+    const int *pair_part;       // [M]
+    const LineDesc *lines;      // device [nlines], lines of this launch
+    int nlines;
+    int nitems;
+    int *queue;                 // work-queue counter (zeroed before launch)
+    long long *stats;           // optional per-launch counters (pairs, pixels, fp64 pixels)
+};
+
+// Host-side launchers (return FSAMD_OK or an error code).
+int launch_fill_i32(int *p, int value, long long n, hipStream_t s);
+int launch_fill_f64(double *p, double value, long long n, hipStream_t s);
+int exclusive_scan_i32_to_i64(const int *in, long long *out, long long n, void *ws, size_t ws_bytes,
+                              hipStream_t s);
+size_t exclusive_scan_ws_bytes(long long n);
+int exclusive_scan_i64_inplace(long long *data, long long n, void *ws, size_t ws_bytes, hipStream_t s);
+
+}  // namespace fsamd

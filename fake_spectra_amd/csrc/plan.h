@@ -1,0 +1,53 @@
+// The binning plan: device neighbour lists and the buffers that own them.
+#pragma once
+#include "fsamd_internal.h"
+
+#include <vector>
+
+namespace fsamd {
+
+// Owning device allocation.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    int alloc(size_t n);
+    void release();
+    void take(DevBuf &o)
+    {
+        release();
+        ptr = o.ptr;
+        bytes = o.bytes;
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    template <typename T>
+    T *as() const { return (T *)ptr; }
+};
+
+int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long long npart,
+               const int *axis, const double *cofm, int nlos, hipStream_t s);
+int plan_near_lines(const fsamd_plan *P, int *out, long long *n_out, hipStream_t s);
+
+}  // namespace fsamd
+
+struct fsamd_plan {
+    double box = 0;
+    long long npart = 0;
+    int nlos = 0;
+    long long npairs = 0;
+    long long max_row = 0;
+    int ncells = 0;
+    fsamd::LosGrid grid{};
+    fsamd::DevBuf axis, cofm;             // copies of the sightline table
+    fsamd::DevBuf cell_start, cell_ids, cell_kl;
+    fsamd::DevBuf poff;                   // [npart+1] particle-major pair offsets
+    fsamd::DevBuf line_off;               // [nlos+1]
+    fsamd::DevBuf pair_part;              // [npairs]
+    fsamd::DevBuf order;                  // sightlines by descending pair count
+    std::vector<int> order_host;
+    std::vector<long long> off_host;
+};

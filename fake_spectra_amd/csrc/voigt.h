@@ -1,0 +1,241 @@
+// Device Voigt profile H(a, u) = Re w(u + i a), w the Faddeeva function.
+//
+// The reference evaluates Re w with the FP64 Faddeeva package
+// (fake_spectra/singleabs.h:56-61 -> Faddeeva.cpp:679-971).  The fused
+// deposit kernel evaluates it in FP32 instead, for damping parameters
+// 0 <= a <= kA32Max, from two expansions:
+//
+//  core, |u| < 7:  H = exp(a^2 - u^2) cos(2au) - (2/sqrt(pi)) [a G(u) - (a^3/6) D'''(u)]
+//     with G = 1 - 2u D(u) = D'(u) (D: Dawson's integral) taken from a
+//     piecewise polynomial table, and D''' = (4u^2 - 6) G + 2.  This is the
+//     Taylor series of w about the real axis, whose even terms sum to
+//     Re exp(-z^2) and whose odd terms are those of the entire function
+//     (2i/sqrt(pi)) D(z).  The a^5 term is below 1e-7 relative for a <= 0.04.
+//  wing, |u| >= 7: the asymptotic series w ~ (i/sqrt(pi)) sum_n (2n-1)!!/2^n z^-(2n+1)
+//     expanded in eps = a/u:  H = (a x / sqrt(pi)) (P(x) - a^2 x Q(x)), x = 1/u^2,
+//     P = sum (2n+1)!!/2^n x^n (9 terms), Q = sum C(2n+3,3) (2n-1)!!/2^n x^n (3 terms).
+//
+// The argument arrives as a double-float pair (uh, ul) because exp(-u^2)
+// amplifies the relative error of u by 2u^2; u^2 and the exponent are then
+// carried in double-float too, and exp uses the hardware 2^f on |f| <= 1/2.
+// Measured against scipy.special.wofz: <= 4e-7 relative for a <= 0.03.
+//
+// An FP64 Faddeeva w(z) (voigt64) backs the FP32 path: pixels whose tail-cut
+// decision sits within a guard band of tautail, and pairs with a > kA32Max.
+// It is accurate to ~1e-13 relative on the path's domain:
+//   real axis: w(x) = exp(-x^2) + (2i/sqrt(pi)) D(x), D by Rybicki's sum;
+//   |z| >= 8 or y >= 2: Laplace continued fraction;
+//   otherwise: Taylor series in y about the real axis.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "dawson_table.h"
+
+namespace fsamd {
+
+constexpr float kVoigtCoreMax = 7.0f;
+constexpr float kA32Max = 0.04f;
+constexpr float kTwoOverSqrtPiF = 1.12837916709551257390f;
+constexpr float kInvSqrtPiF = 0.56418958354775628695f;
+constexpr double kSqrtPi = 1.77245385090551602729;
+constexpr double kTwoOverSqrtPi = 1.12837916709551257390;
+constexpr int kGTableSize = FSAMD_G_NPIECE * (FSAMD_G_DEG + 1);
+
+// (2n+1)!!/2^n, n = 0..8
+#define FSAMD_WING_P {1.0f, 1.5f, 3.75f, 13.125f, 59.0625f, 324.84375f, 2111.484375f, 15836.1328125f, 134607.12890625f}
+// C(2n+3,3) (2n-1)!!/2^n, n = 0..2
+#define FSAMD_WING_Q {1.0f, 5.0f, 26.25f}
+
+// Per-pair constants of the FP32 profile.
+struct VoigtA {
+    float a;            // damping parameter
+    float a2;           // a^2
+    float a2_6;         // a^2 / 6
+    float ea2;          // exp(a^2)
+    float two_a_sqpi;   // 2a/sqrt(pi)
+    float a_sqpi;       // a/sqrt(pi)
+};
+
+__device__ inline VoigtA make_voigt_a(double a)
+{
+    VoigtA v;
+    v.a = (float)a;
+    v.a2 = (float)(a * a);
+    v.a2_6 = (float)(a * a / 6.0);
+    v.ea2 = (float)exp(a * a);
+    v.two_a_sqpi = (float)(a * kTwoOverSqrtPi);
+    v.a_sqpi = (float)(a / kSqrtPi);
+    return v;
+}
+
+// G(u) = 1 - 2u D(u) for 0 <= u < 7 from the table (in LDS or constant memory).
+__device__ inline float g_of_u(float u, const float *tab)
+{
+    int idx = (int)(u * (1.0f / FSAMD_G_WIDTH));
+    idx = idx < FSAMD_G_NPIECE - 1 ? idx : FSAMD_G_NPIECE - 1;
+    const float t = __builtin_fmaf(u, 2.0f / FSAMD_G_WIDTH, -(2.0f * (float)idx + 1.0f));
+    const float *c = tab + idx * (FSAMD_G_DEG + 1);
+    float r = c[FSAMD_G_DEG];
+#pragma unroll
+    for (int k = FSAMD_G_DEG - 1; k >= 0; --k) r = __builtin_fmaf(r, t, c[k]);
+    return r;
+}
+
+// exp(-(p + d)) with p = fl(u^2) and d its double-float remainder, p <= 49.
+__device__ inline float exp_neg_df(float p, float d)
+{
+    const float L2E = 1.44269502f;            // fl(log2 e)
+    const float L2E_LO = 1.925963e-08f;       // log2 e - L2E
+    const float th = p * L2E;
+    float tl = __builtin_fmaf(p, L2E, -th);
+    tl = __builtin_fmaf(p, L2E_LO, tl);
+    tl = __builtin_fmaf(d, L2E, tl);
+    const float n = __builtin_rintf(th);
+    const float f = (th - n) + tl;            // th - n is exact
+    return __builtin_ldexpf(__builtin_amdgcn_exp2f(-f), -(int)n);
+}
+
+// H(a, u) for |u| < 7; (uh, ul) = |u| as a double-float pair, uh >= 0.
+__device__ inline float voigt_core(const VoigtA &v, float uh, float ul, const float *gtab)
+{
+    const float p = uh * uh;
+    const float d = __builtin_fmaf(2.0f * uh, ul, __builtin_fmaf(uh, uh, -p));
+    const float E = exp_neg_df(p, d);
+    const float c = 2.0f * v.a * uh;
+    const float c2 = c * c;
+    float cs = __builtin_fmaf(c2, -1.0f / 40320.0f, 1.0f / 720.0f);
+    cs = __builtin_fmaf(c2, -cs, 1.0f / 24.0f);
+    cs = __builtin_fmaf(c2, -cs, 0.5f);
+    cs = __builtin_fmaf(c2, -cs, 1.0f);
+    const float re_exp = E * v.ea2 * cs;      // Re exp(-z^2)
+    const float G = g_of_u(uh, gtab);
+    const float D3 = __builtin_fmaf(__builtin_fmaf(4.0f, p, -6.0f), G, 2.0f);
+    return __builtin_fmaf(-v.two_a_sqpi, __builtin_fmaf(-v.a2_6, D3, G), re_exp);
+}
+
+// H(a, u) for |u| >= 7 (uh = |u|).
+__device__ inline float voigt_wing(const VoigtA &v, float uh)
+{
+    const float x = __builtin_amdgcn_rcpf(uh * uh);
+    const float P[9] = FSAMD_WING_P;
+    const float Q[3] = FSAMD_WING_Q;
+    float s = P[8];
+#pragma unroll
+    for (int k = 7; k >= 0; --k) s = __builtin_fmaf(s, x, P[k]);
+    float q = __builtin_fmaf(Q[2], x, Q[1]);
+    q = __builtin_fmaf(q, x, Q[0]);
+    return v.a_sqpi * x * __builtin_fmaf(-v.a2 * x, q, s);
+}
+
+// H(a, u) with u given in FP64 (the sample's offset from the line centre in
+// units of b); the FP32 profile math starts from its double-float split.
+__device__ inline float voigt32(const VoigtA &v, double u64, const float *gtab)
+{
+    const double au = u64 < 0 ? -u64 : u64;
+    const float uh = (float)au;
+    if (uh >= kVoigtCoreMax) return voigt_wing(v, uh);
+    const float ul = (float)(au - (double)uh);
+    return voigt_core(v, uh, ul, gtab);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 Faddeeva w(z), y >= 0.
+
+__device__ inline double dawson64(double x)
+{
+    const double ax = fabs(x);
+    if (ax < 0.2) {
+        double t = 1.0, s = 1.0;
+        const double m = -2.0 * ax * ax;
+        for (int k = 1; k < 14; k++) {
+            t *= m / (2 * k + 1);
+            s += t;
+        }
+        return x * s;
+    }
+    const double h = 0.2;
+    const double n0 = 2.0 * floor(0.5 * ax / h + 0.5);
+    const double xp = ax - n0 * h;
+    const double e1 = exp(2.0 * xp * h), e2 = e1 * e1;
+    double up = e1, dn = 1.0 / e1, s = 0.0;
+    for (int i = 0; i < 22; i++) {
+        const double n = 2 * i + 1;
+        const double g = exp(-(n * h) * (n * h));
+        s += g * (up / (n0 + n) + dn / (n0 - n));
+        up *= e2;
+        dn /= e2;
+    }
+    const double d = exp(-xp * xp) * s / kSqrtPi;
+    return x < 0 ? -d : d;
+}
+
+__device__ inline void w64_contfrac(double x, double y, int nterms, double &wr, double &wi)
+{
+    double tr = x, ti = y;
+    for (int k = nterms; k >= 1; k--) {
+        const double den = tr * tr + ti * ti;
+        const double c = 0.5 * k / den;
+        tr = x - c * tr;
+        ti = y + c * ti;
+    }
+    const double den = tr * tr + ti * ti;
+    wr = ti / den / kSqrtPi;
+    wi = tr / den / kSqrtPi;
+}
+
+__device__ inline void w64_taylor_y(double x, double y, double &wr, double &wi)
+{
+    const double w0r = exp(-x * x), w0i = kTwoOverSqrtPi * dawson64(x);
+    const double w1r = -2.0 * x * w0r, w1i = -2.0 * x * w0i + kTwoOverSqrtPi;
+    double sr = w0r - y * w1i, si = w0i + y * w1r;
+    double cr = 0.0, ci = y;
+    double pr = w0r, pim = w0i, qr = w1r, qi = w1i;
+    int small = 0;
+    for (int k = 1; k < 200; k++) {
+        const double nr = -2.0 * x * qr - 2.0 * k * pr;
+        const double ni = -2.0 * x * qi - 2.0 * k * pim;
+        pr = qr;
+        pim = qi;
+        qr = nr;
+        qi = ni;
+        const double t = cr;
+        cr = -ci * y / (k + 1);
+        ci = t * y / (k + 1);
+        const double tr = cr * qr - ci * qi, ti = cr * qi + ci * qr;
+        sr += tr;
+        si += ti;
+        if (fabs(tr) + fabs(ti) <= 1e-19 * (fabs(sr) + fabs(si))) {
+            if (++small >= 3) break;
+        } else {
+            small = 0;
+        }
+    }
+    wr = sr;
+    wi = si;
+}
+
+__device__ inline void w64(double x, double y, double &wr, double &wi)
+{
+    const double ax = fabs(x);
+    double r, i;
+    if (y == 0.0) {
+        r = exp(-ax * ax);
+        i = kTwoOverSqrtPi * dawson64(ax);
+    } else if (ax * ax + y * y >= 64.0) {
+        w64_contfrac(ax, y, 60, r, i);
+    } else if (y >= 2.0) {
+        w64_contfrac(ax, y, 400, r, i);
+    } else {
+        w64_taylor_y(ax, y, r, i);
+    }
+    wr = r;
+    wi = x < 0 ? -i : i;
+}
+
+__device__ inline double voigt64(double a, double u)
+{
+    double r, i;
+    w64(u, a, r, i);
+    return r;
+}
+
+}  // namespace fsamd

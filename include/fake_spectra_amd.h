@@ -1,0 +1,103 @@
+/* fake_spectra_amd: C ABI of the MI355X (gfx950) sightline spectral-synthesis
+ * library.  Every pointer argument documented "device" is a HIP device
+ * pointer; "stream" is a hipStream_t passed as void* (0 = null stream).
+ * All functions return FSAMD_OK (0) or an error code; fsamd_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * The entry points replace the reference's native path behind
+ * Spectra.get_tau / Spectra.get_col_density:
+ *   fsamd_plan_create        IndexTable ctor + get_near_particles
+ *                            (fake_spectra/index_table.cpp:7-18, 130-150)
+ *   fsamd_plan_near_lines    _near_lines (fake_spectra/py_module.cpp:25-99)
+ *   fsamd_tau                ParticleInterp::compute_tau (part_int.cpp:20-51)
+ *                            for several lines at once
+ *   fsamd_colden             ParticleInterp::compute_colden (part_int.cpp:53-84)
+ *   fsamd_particle_interpolate  _Particle_Interpolate (py_module.cpp:103-233)
+ */
+#ifndef FAKE_SPECTRA_AMD_H
+#define FAKE_SPECTRA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSAMD_OK 0
+#define FSAMD_ERR_ARG 1          /* invalid argument (ValueError in Python) */
+#define FSAMD_ERR_OOM 2          /* device allocation failed (MemoryError) */
+#define FSAMD_ERR_HIP 3          /* HIP runtime error */
+#define FSAMD_ERR_UNSUPPORTED 4  /* unsupported kernel id or size */
+
+#define FSAMD_MODE_BATCHED 0     /* all sightlines x lines in one persistent launch */
+#define FSAMD_MODE_PLAIN 1       /* one launch per sightline and line */
+
+const char *fsamd_last_error(void);
+int fsamd_version(void);
+
+/* Sightline <-> particle binning: the on-device counterpart of IndexTable.
+ * Holds, for every sightline, the indices of particles whose kernel support
+ * reaches it (index_table.cpp:22-50 acceptance test, applied exactly),
+ * ascending within each sightline. */
+typedef struct fsamd_plan fsamd_plan;
+
+/* Builds the neighbour lists.  pos [npart,3] f32, h [npart] f32, axis [nlos]
+ * i32 with values 1..3, cofm [nlos,3] f64: all device.  Synchronizes the
+ * stream once to size the pair list.  Replaces IndexTable::IndexTable and
+ * IndexTable::get_near_particles (index_table.cpp:7-18, 130-150). */
+int fsamd_plan_create(double box, const float *pos, const float *h, long long npart,
+                      const int *axis, const double *cofm, int nlos, void *stream,
+                      fsamd_plan **out);
+int fsamd_plan_destroy(fsamd_plan *plan);
+long long fsamd_plan_npairs(const fsamd_plan *plan);
+/* Device pointers to the lists: offsets [nlos+1] (int64), particle ids [npairs]. */
+int fsamd_plan_pairs(const fsamd_plan *plan, const long long **offsets, const int **parts);
+/* Sorted unique ids of particles with at least one nearby sightline, written
+ * to out (device, capacity npart); *n_out receives the count (host).
+ * Replaces Py_near_lines (py_module.cpp:25-99). */
+int fsamd_plan_near_lines(const fsamd_plan *plan, int *out, long long *n_out, void *stream);
+
+/* One absorption line of a multi-line call: the (lambda, gamma, fosc,
+ * amumass) quadruple of _Particle_Interpolate with its species density. */
+typedef struct fsamd_line {
+    double lambda_cm;   /* rest wavelength [cm] */
+    double gamma;       /* damping constant [1/s] */
+    double fosc;        /* oscillator strength */
+    double amumass;     /* species mass [amu] */
+    const float *dens;  /* device [npart] species density */
+    double *out;        /* device [nlos, nbins] float64; fully overwritten */
+} fsamd_line;
+
+/* Optical depth of nlines lines (ParticleInterp::compute_tau,
+ * part_int.cpp:20-51, with LineAbsorption::add_tau_particle,
+ * absorption.cpp:212-279).  kernel: 0 tophat, 1 cubic spline.
+ * mode: FSAMD_MODE_BATCHED or FSAMD_MODE_PLAIN. */
+int fsamd_tau(const fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail,
+              const float *pos, const float *vel, const float *temp, const float *h,
+              const fsamd_line *lines, int nlines, int mode, void *stream);
+
+/* Column density (ParticleInterp::compute_colden, part_int.cpp:53-84, with
+ * LineAbsorption::add_colden_particle, absorption.cpp:167-210).  Only dens
+ * and out of each line are used. */
+int fsamd_colden(const fsamd_plan *plan, int nbins, int kernel, double velfac, const float *pos,
+                 const float *h, const fsamd_line *lines, int nlines, void *stream);
+
+/* Drop-in of _Particle_Interpolate (py_module.cpp:103-233) on device
+ * pointers: out is device [nlos, nbins] float64.  vel and temp are unused
+ * when compute_tau == 0.  atime is accepted and unused, as in the
+ * reference (absorption.cpp:157). */
+int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double box100,
+                               double velfac, double atime, double lambda, double gamma,
+                               double fosc, double amumass, double tautail, const float *pos,
+                               const float *vel, const float *dens, const float *temp,
+                               const float *h, long long npart, const int *axis,
+                               const double *cofm, int nlos, double *out, void *stream);
+
+/* Counters of the last fsamd_tau call on this thread (host): pairs deposited,
+ * pixels deposited, pixels re-evaluated in FP64 at the tail cut, pairs
+ * evaluated entirely in FP64.  Filled only when fsamd_set_stats(1). */
+int fsamd_set_stats(int enable);
+int fsamd_last_stats(long long *out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

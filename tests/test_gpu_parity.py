@@ -1,0 +1,140 @@
+"""GPU path against the oracle and the reference's golden outputs.
+Binning is index work: bit-exact.  tau/colden: the stated tolerances."""
+import numpy as np
+import pytest
+import torch
+
+from fake_spectra_amd import spectra_priv as SP, synth
+from fake_spectra_amd.lines import HI_1215, HI_1025, CIV_1548
+from golden_cases import c0_fixture, crafted, tau_keys
+from gpu_util import dev, assert_tau_close, assert_colden_close, plan_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _pairs(plan):
+    off, parts = plan.pairs()
+    return off.cpu().numpy(), parts.cpu().numpy()
+
+
+def test_binning_c0_matches_reference():
+    w, z = c0_fixture()
+    sel = z["sel"]
+    plan = plan_for(w, w.axis[sel], w.cofm[sel])
+    off, parts = _pairs(plan)
+    assert np.array_equal(off, z["offsets"])
+    assert np.array_equal(parts, z["parts"])
+    near = SP._near_lines(w.box, w.pos, w.hh, w.axis, w.cofm)
+    assert np.array_equal(near, z["near_lines"])
+
+
+def test_binning_crafted_matches_reference():
+    cases, z = crafted()
+    for name, c in cases.items():
+        plan = SP.Plan(float(c["box"]), dev(c["pos"]), dev(c["hh"]), dev(c["axis"]), dev(c["cofm"]))
+        off, parts = _pairs(plan)
+        assert np.array_equal(off, z[name + "_offsets"]), name
+        assert np.array_equal(parts, z[name + "_parts"]), name
+        near = SP._near_lines(float(c["box"]), c["pos"], c["hh"], c["axis"], c["cofm"])
+        assert np.array_equal(near, z[name + "_near"]), name
+
+
+@pytest.mark.parametrize("idx,scale,nlos", [(1, 1 / 64, 24), (2, 1 / 64, None), (4, 1 / 512, 300)])
+def test_binning_configs_match_oracle(oracle, idx, scale, nlos):
+    w = synth.config(idx, scale=scale, nlos=nlos)
+    plan = plan_for(w)
+    off, parts = _pairs(plan)
+    roff, rparts, _ = oracle.near_particles(w.box, w.pos, w.hh, w.axis, w.cofm)
+    assert np.array_equal(off, roff) and np.array_equal(parts, rparts)
+    assert plan.npairs == len(rparts)
+
+
+def test_tau_c0_full(oracle):
+    w, _ = c0_fixture()
+    args = w.interp_args(HI_1215, True)
+    got = SP._Particle_Interpolate(*args)
+    assert_tau_close(got, oracle.interpolate(*args), "c0 tau")
+
+
+def test_colden_c0_full(oracle):
+    w, z = c0_fixture()
+    args = w.interp_args(None, False)
+    got = SP._Particle_Interpolate(*args)
+    assert_colden_close(got, oracle.interpolate(*args), "c0 colden")
+    sel = z["sel"]
+    assert_colden_close(got[sel], z["colden"], "c0 colden vs reference")
+
+
+def test_tau_c0_vs_reference_fixture():
+    w, z = c0_fixture()
+    args = list(w.interp_args(HI_1215, True))
+    args[16], args[17] = w.axis[z["sel"]], w.cofm[z["sel"]]
+    assert_tau_close(SP._Particle_Interpolate(*args), z["tau"], "c0 subset vs reference")
+
+
+def test_crafted_cases_vs_reference_fixture():
+    cases, z = crafted()
+    velfac = float(z["velfac"])
+    for key, case, kern, line, gs, nb in tau_keys(z):
+        c = cases[case]
+        got = SP._Particle_Interpolate(1, nb, kern, float(c["box"]), velfac, 0.25, line.lambda_cm, line.gamma * gs,
+                                       line.fosc, line.amumass, 1e-7, c["pos"], c["vel"], c["dens"], c["temp"],
+                                       c["hh"], c["axis"], c["cofm"])
+        assert_tau_close(got, z[key], key)
+    for case, c in cases.items():
+        for kern in (0, 1):
+            got = SP._Particle_Interpolate(0, 257, kern, float(c["box"]), velfac, 0.25, HI_1215.lambda_cm,
+                                           HI_1215.gamma, HI_1215.fosc, HI_1215.amumass, 1e-7, c["pos"],
+                                           np.zeros(1, np.float32), c["dens"], np.zeros(1, np.float32), c["hh"],
+                                           c["axis"], c["cofm"])
+            assert_colden_close(got, z["%s_k%d_colden" % (case, kern)], case)
+
+
+@pytest.mark.parametrize("idx,scale,nlos", [(1, 1 / 64, 16), (2, 1 / 64, 6), (3, 1 / 8, 16), (4, 1 / 512, 64)])
+def test_tau_configs_vs_oracle(oracle, idx, scale, nlos):
+    """Every line of each workload, from one binning, in one batched launch."""
+    w = synth.config(idx, scale=scale, nlos=nlos)
+    plan = plan_for(w)
+    outs = SP.tau(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), w.lines,
+                  [dev(w.dens_for(ln)) for ln in w.lines], w.tautail)
+    for ln, o in zip(w.lines, outs):
+        ref = oracle.interpolate(*w.interp_args(ln, True))
+        assert_tau_close(o.cpu().numpy(), ref, "c%d %s" % (idx, ln.name))
+    col = SP.colden(plan, w.nbins, w.kernel, w.velfac, [dev(w.dens_for(w.lines[0]))])[0]
+    assert_colden_close(col.cpu().numpy(), oracle.interpolate(*w.interp_args(None, False)), "c%d colden" % idx)
+
+
+def test_plain_mode_equals_batched():
+    w = synth.config(2, scale=1 / 64, nlos=4)
+    plan = plan_for(w)
+    dens = [dev(w.dens_for(ln)) for ln in w.lines]
+    a = SP.tau(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), w.lines, dens, mode=0)
+    b = SP.tau(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), w.lines, dens, mode=1)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(x.cpu().numpy(), y.cpu().numpy(), rtol=1e-12, atol=1e-300)
+
+
+def test_rows_wider_than_lds(oracle):
+    """nbins too large for an LDS row: deposits go straight to HBM."""
+    w = synth.config(0, nlos=3, n_side=12)
+    w.nbins = 40000
+    args = w.interp_args(HI_1215, True)
+    assert_tau_close(SP._Particle_Interpolate(*args), oracle.interpolate(*args), "nbins=40000")
+    cargs = w.interp_args(None, False)
+    assert_colden_close(SP._Particle_Interpolate(*cargs), oracle.interpolate(*cargs), "nbins=40000 colden")
+
+
+def test_cold_gas_uses_fp64_profile(oracle):
+    """Damping parameters above the FP32 profile's range go through FP64."""
+    w = synth.config(0, nlos=6, n_side=10)
+    w.temp[:] = np.float32(1.5)        # H I at 1.5 K: a ~ 0.04
+    w.temp[::2] = np.float32(1.0)
+    args = w.interp_args(HI_1215, True)
+    SP.set_stats(True)
+    try:
+        got = SP._Particle_Interpolate(*args)
+        st = SP.last_stats()
+    finally:
+        SP.set_stats(False)
+    assert st["fp64_pairs"] > 0
+    assert_tau_close(got, oracle.interpolate(*args), "cold gas")
