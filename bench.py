@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Throughput of the sightline spectral-synthesis path on one MI355X.
+
+A step is one pass of the hot path over the workload: build the
+sightline<->particle neighbour lists (IndexTable) from device-resident
+particles, then deposit the optical depth of every line of every sightline
+in one batched launch.  The default workload is 256^3 particles in a
+60 Mpc/h box, 5000 sightlines of 2048 pixels, H I 1215 and H I 1025
+(synthetic gas field, seeded).  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--config 1|2|...] [--no-cpu] [--no-plain]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(w, budget_s):
+    """The reference C++ (oracle/_ref, OpenMP, all host threads) on every k-th
+    sightline of the same workload with the full particle set: first line only,
+    grown until about budget_s of CPU time.  Falls back to the oracle's C
+    restatement where the reference library is not built."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    lib = O.load_ref() if kind == "reference" else O.load_oracle()
+    cores = lib.max_threads()
+    line = w.lines[0]
+    nsel = 4
+    t = 0.0
+    while True:
+        k = max(1, w.nlos // nsel)
+        sel = np.arange(0, w.nlos, k)[:nsel]
+        args = list(w.interp_args(line, True))
+        args[16], args[17] = w.axis[sel], w.cofm[sel]
+        t0 = time.perf_counter()
+        lib.interpolate(*args)
+        t = time.perf_counter() - t0
+        if t >= budget_s / 3 or nsel >= w.nlos:
+            break
+        nsel = min(w.nlos, max(nsel * 2, int(nsel * budget_s / 3 / max(t, 1e-3))))
+    value = len(sel) * w.nbins / t
+    return {"value": value, "unit": "sightline-pixels/s", "cores": cores, "kind": kind,
+            "sample": "%s line %s, %d of %d sightlines (every %d-th), all %d particles, index table + deposit, "
+                      "%.1f s" % ("reference C++ -O2 -ffast-math OpenMP" if kind == "reference" else "C restatement",
+                                  line.name, len(sel), w.nlos, k, w.npart, t)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each sightline is independent; "
+              "see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from fake_spectra_amd import synth, spectra_priv as SP
+    from fake_spectra_amd import _native as N
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    w = synth.config(args.config, scale=args.scale)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    pos, hh, ax, cofm = dev(w.pos), dev(w.hh), dev(w.axis), dev(w.cofm)
+    vel, temp = dev(w.vel), dev(w.temp)
+    dens = [dev(w.dens_for(ln)) for ln in w.lines]
+    outs = [torch.empty((w.nlos, w.nbins), dtype=torch.float64, device="cuda") for _ in w.lines]
+    K = len(w.lines)
+    torch.cuda.synchronize()
+
+    state = {}
+
+    def step():
+        plan = SP.Plan(w.box, pos, hh, ax, cofm)
+        SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail, out=outs)
+        st = SP.last_stats()
+        state["npairs"] = plan.npairs
+        state.setdefault("ms", []).append(st["gpu_ms"])
+        state["stats"] = st
+        plan.close()
+
+    SP.set_stats(True)
+    for _ in range(args.warmup):
+        step()
+    state["ms"] = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    SP.set_stats(False)
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    units = K * w.nlos * w.nbins
+    value = units / (elapsed / args.steps)
+    dep_ms = float(np.mean(state["ms"]))
+    M = state["npairs"]
+    st = state["stats"]
+    # algorithmic HBM bytes of one deposit launch: per (pair, line) the pair
+    # index (4 B) and the particle gather pos/vel/T/h/dens (28 B); per
+    # sightline-pixel the float64 output (8 B)
+    dep_bytes = 32.0 * M * K + 8.0 * units
+    achieved = dep_bytes / (dep_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "tau_deposit_c%d_traffic.json" % args.config)
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    peak = C_double = None
+    import ctypes
+    C_double = ctypes.c_double()
+    N.check(N.lib().fsamd_voigt_peak(64.0, 5e-4, ctypes.byref(C_double), ctypes.c_void_p(0)))
+    peak = C_double.value
+    samples_per_s = st["voigt_samples"] / (dep_ms * 1e-3)
+
+    out = {
+        "metric": "sightline-pixels/sec (tau fwd, fwd+adjoint) at stated tol, batched vs plain, 1 GPU",
+        "value": value,
+        "unit": "sightline-pixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 profile / f64 mapping+accumulation",
+        "data": "synthetic (seeded lognormal gas field)",
+        "config": {"workload": "c%d: %d^3 particles, %d sightlines x %d pixels, lines %s, kernel %s, batched tau "
+                               "(binning included)" % (args.config, round(w.npart ** (1 / 3)), w.nlos, w.nbins,
+                                                       "+".join(ln.name for ln in w.lines),
+                                                       "cubic" if w.kernel == 1 else "tophat"),
+                   "global_batch": w.nlos, "seq_len": w.nbins, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "kernel": "tau_kernel (fused deposit)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes_per_launch": dep_bytes, "launch_ms": dep_ms},
+        "valu": {"kernel": "tau_kernel", "achieved_samples_per_s": samples_per_s,
+                 "isolated_peak_samples_per_s": peak, "frac": samples_per_s / peak,
+                 "voigt_samples_per_launch": st["voigt_samples"],
+                 "pixels_deposited": st["pixels"], "pixels_evaluated": st["evaluated_pixels"],
+                 "fp64_tailcut_pixels": st["fp64_pixels"], "fp64_pairs": st["fp64_pairs"]},
+        "pairs": M,
+        "deposit_ms": dep_ms,
+        "binning_ms": ms_per_step - dep_ms,
+    }
+    if not args.no_plain:
+        plan = SP.Plan(w.box, pos, hh, ax, cofm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail, out=outs, mode=N.MODE_PLAIN)
+        torch.cuda.synchronize()
+        tplain = time.perf_counter() - t0
+        tbatch = dep_ms * 1e-3
+        out["plain"] = {"value": units / tplain, "deposit_s": tplain,
+                        "batched_over_plain": tplain / tbatch,
+                        "note": "deposit only, same binning; one launch per sightline and line"}
+        plan.close()
+    if not args.no_cpu:
+        cb = cpu_baseline(w, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -23,7 +23,8 @@ MODE_PLAIN = 1
 # Every symbol include/fake_spectra_amd.h declares.
 EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan_destroy",
            "fsamd_plan_npairs", "fsamd_plan_pairs", "fsamd_plan_near_lines", "fsamd_tau",
-           "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats")
+           "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats", "fsamd_last_ms",
+           "fsamd_voigt_peak")
 
 
 class FsamdLine(C.Structure):
@@ -62,6 +63,10 @@ def _declare(L):
     L.fsamd_set_stats.restype = i
     L.fsamd_last_stats.argtypes = [C.POINTER(ll)]
     L.fsamd_last_stats.restype = i
+    L.fsamd_last_ms.argtypes = []
+    L.fsamd_last_ms.restype = d
+    L.fsamd_voigt_peak.argtypes = [d, d, C.POINTER(d), vp]
+    L.fsamd_voigt_peak.restype = i
 
 
 def lib():

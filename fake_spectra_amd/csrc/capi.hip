@@ -11,8 +11,9 @@
 namespace fsamd {
 
 static thread_local std::string g_error;
-static thread_local long long g_stats[4] = {0, 0, 0, 0};
+static thread_local long long g_stats[kNStats] = {0, 0, 0, 0, 0, 0};
 static thread_local int g_stats_on = 0;
+static thread_local double g_last_ms = 0;
 
 void set_error(const std::string &msg) { g_error = msg; }
 
@@ -50,13 +51,20 @@ static void line_consts(const fsamd_line &l, LineDesc &d)
     d.out = l.out;
 }
 
-// Work items for lines [k0, k0+K) in the plan's heaviest-first order; a
-// sightline with more than `chunk` pairs is split into non-direct items.
-static void build_items(const fsamd_plan *P, int K, long long chunk, std::vector<WorkItem> &items,
-                        bool &any_split)
+// Work items for K lines in the plan's heaviest-first order; a sightline
+// with more than `chunk` pairs is split into non-direct items.  Cached.
+static int item_set(fsamd_plan *P, int K, long long chunk, fsamd_plan::ItemSet **out)
 {
-    items.clear();
-    any_split = false;
+    for (auto &it : P->item_sets)
+        if (it->K == K && it->chunk == chunk) {
+            *out = it.get();
+            return FSAMD_OK;
+        }
+    auto set = std::make_unique<fsamd_plan::ItemSet>();
+    set->K = K;
+    set->chunk = chunk;
+    set->split = false;
+    std::vector<WorkItem> &items = set->host;
     for (int s : P->order_host) {
         const long long a = P->off_host[s], b = P->off_host[s + 1];
         const long long n = b - a;
@@ -64,15 +72,51 @@ static void build_items(const fsamd_plan *P, int K, long long chunk, std::vector
             if (n <= chunk) {
                 items.push_back(WorkItem{s, k, a, b, 1, 0});
             } else {
-                any_split = true;
+                set->split = true;
                 for (long long c = a; c < b; c += chunk)
                     items.push_back(WorkItem{s, k, c, std::min(b, c + chunk), 0, 0});
             }
         }
     }
+    if (items.size() >= (size_t)INT32_MAX) return fail(FSAMD_ERR_ARG, "too many work items");
+    int rc = set->buf.alloc(sizeof(WorkItem) * std::max<size_t>(items.size(), 1));
+    if (rc) return rc;
+    if (!items.empty())
+        FSAMD_HIP(hipMemcpy(set->buf.ptr, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
+    P->item_sets.push_back(std::move(set));
+    *out = P->item_sets.back().get();
+    return FSAMD_OK;
 }
 
-static int run_deposit(bool tau, const fsamd_plan *P, int nbins, int kernel, double velfac, double tautail,
+static int grow_fixups(fsamd_plan *P, long long cap, hipStream_t s)
+{
+    cap = std::min<long long>(cap, INT32_MAX / 2);
+    FSAMD_HIP(hipStreamSynchronize(s));
+    int rc = P->fix.alloc(sizeof(Fixup) * (size_t)cap);
+    if (rc) return rc;
+    P->fix_cap = (int)cap;
+    return FSAMD_OK;
+}
+
+// Upload the line table only when it changed since the last call.
+static int upload_lines(fsamd_plan *P, const std::vector<LineDesc> &ld, hipStream_t s)
+{
+    const size_t bytes = sizeof(LineDesc) * ld.size();
+    if (P->lines_dev.bytes < bytes) {
+        FSAMD_HIP(hipStreamSynchronize(s));
+        int rc = P->lines_dev.alloc(bytes);
+        if (rc) return rc;
+        P->lines_host.clear();
+    }
+    if (P->lines_host.size() == ld.size() && std::memcmp(P->lines_host.data(), ld.data(), bytes) == 0)
+        return FSAMD_OK;
+    FSAMD_HIP(hipStreamSynchronize(s));   // the previous table may still be in use
+    FSAMD_HIP(hipMemcpy(P->lines_dev.ptr, ld.data(), bytes, hipMemcpyHostToDevice));
+    P->lines_host = ld;
+    return FSAMD_OK;
+}
+
+static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double velfac, double tautail,
                        const float *pos, const float *vel, const float *temp, const float *h,
                        const fsamd_line *lines, int nlines, int mode, hipStream_t s)
 {
@@ -83,20 +127,34 @@ static int run_deposit(bool tau, const fsamd_plan *P, int nbins, int kernel, dou
     if (nlines < 1 || !lines) return fail(FSAMD_ERR_ARG, "at least one line is required");
     if (mode != FSAMD_MODE_BATCHED && mode != FSAMD_MODE_PLAIN) return fail(FSAMD_ERR_ARG, "unknown mode");
     if ((long long)P->nlos * nbins >= (1ll << 62)) return fail(FSAMD_ERR_ARG, "output too large");
-    DeviceInfo dev;
-    int rc = device_info(dev);
-    if (rc) return rc;
-    const bool lds = deposit_lds_bytes(nbins) + 2048 <= (size_t)dev.max_lds;
     for (int i = 0; i < nlines; ++i)
         if (!lines[i].out || !lines[i].dens) return fail(FSAMD_ERR_ARG, "line without dens or out");
+    static thread_local DeviceInfo dev;
+    int rc;
+    if (dev.ncu == 0 && (rc = device_info(dev))) return rc;
+    const bool lds = deposit_lds_bytes(nbins) + 2048 <= (size_t)dev.max_lds;
+    const int ngroups = (nlines + kMaxLines - 1) / kMaxLines;
 
-    DevBuf queue, stats;
-    if ((rc = queue.alloc(sizeof(int) * 4))) return rc;
+    std::vector<LineDesc> ld(nlines);
+    for (int i = 0; i < nlines; ++i) line_consts(lines[i], ld[i]);
+    if ((rc = upload_lines(P, ld, s))) return rc;
+    if (P->queue.bytes < sizeof(int) * (size_t)ngroups && (rc = P->queue.alloc(sizeof(int) * ngroups))) return rc;
     if (g_stats_on) {
-        if ((rc = stats.alloc(sizeof(long long) * 4))) return rc;
-        FSAMD_HIP(hipMemsetAsync(stats.ptr, 0, sizeof(long long) * 4, s));
+        if (!P->stats.ptr && (rc = P->stats.alloc(sizeof(long long) * kNStats))) return rc;
+        if ((rc = launch_fill_f64((double *)P->stats.ptr, 0.0, kNStats, s))) return rc;   // all-zero bits
+        for (auto &e : P->ev)
+            if (!e) FSAMD_HIP(hipEventCreate(&e));
+        FSAMD_HIP(hipEventRecord(P->ev[0], s));
     }
-    for (int k0 = 0; k0 < nlines; k0 += kMaxLines) {
+    const bool plain = (mode == FSAMD_MODE_PLAIN);
+    if (tau) {
+        // room for deferred FP64 walks: a few percent of pair-directions
+        const long long want = std::max<long long>(65536, 2 * P->npairs * std::min(nlines, kMaxLines) / 32);
+        if (P->fix_cap < want && (rc = grow_fixups(P, want, s))) return rc;
+        if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(int) * 2))) return rc;
+    }
+    for (int g = 0; g < ngroups; ++g) {
+        const int k0 = g * kMaxLines;
         const int K = std::min(kMaxLines, nlines - k0);
         DepositArgs A;
         std::memset(&A, 0, sizeof(A));
@@ -115,51 +173,59 @@ static int run_deposit(bool tau, const fsamd_plan *P, int nbins, int kernel, dou
         A.line_off = P->line_off.as<long long>();
         A.pair_part = P->pair_part.as<int>();
         A.nlines = K;
-        std::vector<LineDesc> ldesc(K);
-        for (int k = 0; k < K; ++k) line_consts(lines[k0 + k], ldesc[k]);
-        DevBuf dlines;
-        if ((rc = dlines.alloc(sizeof(LineDesc) * K))) return rc;
-        FSAMD_HIP(hipMemcpyAsync(dlines.ptr, ldesc.data(), sizeof(LineDesc) * K, hipMemcpyHostToDevice, s));
-        A.lines = dlines.as<LineDesc>();
-        A.queue = queue.as<int>();
-        A.stats = g_stats_on ? stats.as<long long>() : nullptr;
-
-        const bool plain = (mode == FSAMD_MODE_PLAIN);
-        std::vector<WorkItem> items;
-        bool split = false;
-        build_items(P, K, plain ? kPlainChunk : (1ll << 62), items, split);
-        if (split || !lds) {
-            for (int k = 0; k < K; ++k)
-                if ((rc = launch_fill_f64(ldesc[k].out, 0.0, (long long)P->nlos * nbins, s))) return rc;
-        }
-        if (items.empty()) continue;
-        DevBuf ditems;
-        if ((rc = ditems.alloc(sizeof(WorkItem) * items.size()))) return rc;
-        FSAMD_HIP(hipMemcpyAsync(ditems.ptr, items.data(), sizeof(WorkItem) * items.size(),
-                                 hipMemcpyHostToDevice, s));
-        if (!plain) {
-            A.nitems = (int)items.size();
-            if ((rc = launch_fill_i32(A.queue, 0, 1, s))) return rc;
-            const int per_cu = deposit_blocks_per_cu(tau, lds, nbins);
-            const int nblocks = (int)std::min<long long>((long long)per_cu * dev.ncu, A.nitems);
-            if ((rc = launch_deposit(tau, lds, A, ditems.as<WorkItem>(), 0, nblocks, 1, s))) return rc;
-        } else {
-            // one launch per (sightline, line), in sightline index order
-            std::vector<int> first(P->nlos * (size_t)K + 1, -1);
-            size_t i = 0;
-            while (i < items.size()) {
-                size_t j = i;
-                while (j < items.size() && items[j].los == items[i].los && items[j].line == items[i].line) ++j;
-                A.nitems = (int)(j - i);
-                if ((rc = launch_deposit(tau, lds, A, ditems.as<WorkItem>(), (int)i, A.nitems, 0, s))) return rc;
-                i = j;
+        A.lines = P->lines_dev.as<LineDesc>() + k0;
+        A.queue = P->queue.as<int>() + g;
+        A.stats = g_stats_on ? P->stats.as<long long>() : nullptr;
+        fsamd_plan::ItemSet *set = nullptr;
+        if ((rc = item_set(P, K, plain ? kPlainChunk : (1ll << 62), &set))) return rc;
+        const std::vector<WorkItem> &items = set->host;
+        for (int attempt = 0;; ++attempt) {
+            A.fix = P->fix.as<Fixup>();
+            A.fix_cap = P->fix_cap;
+            A.fix_count = P->fix_count.as<int>();
+            if (set->split || !lds) {
+                for (int k = 0; k < K; ++k)
+                    if ((rc = launch_fill_f64(ld[k0 + k].out, 0.0, (long long)P->nlos * nbins, s))) return rc;
             }
+            if (items.empty()) break;
+            if (tau && (rc = launch_fill_i32(A.fix_count, 0, 2, s))) return rc;
+            if (!plain) {
+                A.nitems = (int)items.size();
+                if ((rc = launch_fill_i32(A.queue, 0, 1, s))) return rc;
+                const int per_cu = deposit_blocks_per_cu(tau, lds, nbins);
+                const int nblocks = (int)std::min<long long>((long long)per_cu * dev.ncu, A.nitems);
+                if ((rc = launch_deposit(tau, lds, A, set->buf.as<WorkItem>(), 0, nblocks, 1, s))) return rc;
+            } else {
+                // one launch per (sightline, line), in the plan's item order
+                size_t i = 0;
+                while (i < items.size()) {
+                    size_t j = i;
+                    while (j < items.size() && items[j].los == items[i].los && items[j].line == items[i].line) ++j;
+                    A.nitems = (int)(j - i);
+                    if ((rc = launch_deposit(tau, lds, A, set->buf.as<WorkItem>(), (int)i, A.nitems, 0, s)))
+                        return rc;
+                    i = j;
+                }
+            }
+            if (!tau) break;
+            if ((rc = launch_tau_fixup(A, set->buf.as<WorkItem>(), dev.ncu * 2, s))) return rc;
+            int fc[2] = {0, 0};
+            FSAMD_HIP(hipMemcpyAsync(fc, A.fix_count, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+            FSAMD_HIP(hipStreamSynchronize(s));
+            if (!fc[1]) break;
+            // the list overflowed: grow it and run this line group again
+            if (attempt > 2) return fail(FSAMD_ERR_HIP, "fix-up list overflow persists");
+            if ((rc = grow_fixups(P, 2ll * fc[0] + 65536, s))) return rc;
+            if (g_stats_on && (rc = launch_fill_f64((double *)P->stats.ptr, 0.0, kNStats, s))) return rc;
         }
-        FSAMD_HIP(hipStreamSynchronize(s));   // items / queue buffers are freed on return
     }
     if (g_stats_on) {
-        FSAMD_HIP(hipMemcpyAsync(g_stats, stats.ptr, sizeof(long long) * 4, hipMemcpyDeviceToHost, s));
+        FSAMD_HIP(hipEventRecord(P->ev[1], s));
+        FSAMD_HIP(hipMemcpyAsync(g_stats, P->stats.ptr, sizeof(long long) * kNStats, hipMemcpyDeviceToHost, s));
         FSAMD_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        FSAMD_HIP(hipEventElapsedTime(&ms, P->ev[0], P->ev[1]));
+        g_last_ms = ms;
     }
     return FSAMD_OK;
 }
@@ -217,7 +283,7 @@ int fsamd_plan_near_lines(const fsamd_plan *plan, int *out, long long *n_out, vo
     return plan_near_lines(plan, out, n_out, (hipStream_t)stream);
 }
 
-int fsamd_tau(const fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail, const float *pos,
+int fsamd_tau(fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail, const float *pos,
               const float *vel, const float *temp, const float *h, const fsamd_line *lines, int nlines,
               int mode, void *stream)
 {
@@ -225,7 +291,7 @@ int fsamd_tau(const fsamd_plan *plan, int nbins, int kernel, double velfac, doub
                        (hipStream_t)stream);
 }
 
-int fsamd_colden(const fsamd_plan *plan, int nbins, int kernel, double velfac, const float *pos, const float *h,
+int fsamd_colden(fsamd_plan *plan, int nbins, int kernel, double velfac, const float *pos, const float *h,
                  const fsamd_line *lines, int nlines, void *stream)
 {
     return run_deposit(false, plan, nbins, kernel, velfac, 0.0, pos, nullptr, nullptr, h, lines, nlines,
@@ -264,10 +330,43 @@ int fsamd_set_stats(int enable)
     return FSAMD_OK;
 }
 
-int fsamd_last_stats(long long *out4)
+int fsamd_last_stats(long long *out6)
 {
-    if (!out4) return fail(FSAMD_ERR_ARG, "NULL argument");
-    for (int i = 0; i < 4; ++i) out4[i] = g_stats[i];
+    if (!out6) return fail(FSAMD_ERR_ARG, "NULL argument");
+    for (int i = 0; i < kNStats; ++i) out6[i] = g_stats[i];
+    return FSAMD_OK;
+}
+
+double fsamd_last_ms(void) { return g_last_ms; }
+
+int fsamd_voigt_peak(double umax, double a, double *samples_per_s, void *stream)
+{
+    if (!samples_per_s || !(umax > 0)) return fail(FSAMD_ERR_ARG, "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    DeviceInfo dev;
+    int rc = device_info(dev);
+    if (rc) return rc;
+    DevBuf sink;
+    if ((rc = sink.alloc(16))) return rc;
+    const int nblocks = dev.ncu * 8;
+    const long long per_thread = 4096;
+    const double du = 0.001;
+    hipEvent_t e0, e1;
+    FSAMD_HIP(hipEventCreate(&e0));
+    FSAMD_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        FSAMD_HIP(hipEventRecord(e0, s));
+        if ((rc = launch_voigt_peak(per_thread, nblocks, du, umax, (float)a, sink.as<float>(), s))) return rc;
+        FSAMD_HIP(hipEventRecord(e1, s));
+        FSAMD_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        FSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *samples_per_s = (double)nblocks * 256 * per_thread / (best * 1e-3);
     return FSAMD_OK;
 }
 

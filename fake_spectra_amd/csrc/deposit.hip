@@ -38,6 +38,18 @@ __constant__ float c_gtab[kGTableSize] = FSAMD_G_COEF_INIT;
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr double kGuard = 2e-5;
+constexpr int kWingTerms = 10;
+
+// Far-wing coefficient tables.  The profile's asymptotic series in x = 1/u^2,
+//   H(a,u) = (a/sqrt(pi)) sum_k (p_k - a^2 q_{k-1}) u^(-2k-2),
+// p_k = (2k+1)!!/2^k, q_k = C(2k+3,3) (2k-1)!!/2^k (voigt.h), is expanded
+// about the pixel centre U over the sample offsets c (Taylor in c/U); the
+// even moments m_j = sum w c^(2j) of the pixel's weighted samples then give
+//   sum_samples w H = sum_n X^(n+1) sum_{k<=n} g_k C(2n+1, 2(n-k)) m_(n-k).
+__constant__ double c_wing_p[kWingTerms] = {1.0, 1.5, 3.75, 13.125, 59.0625, 324.84375, 2111.484375,
+                                             15836.1328125, 134607.12890625, 1278767.724609375};
+__constant__ double c_wing_q[kWingTerms] = {1.0, 5.0, 26.25, 157.5, 1082.8125, 8445.9375, 73901.953125,
+                                             717904.6875, 7672606.34765625, 89513740.72265625};
 
 // singleabs.h:17-26
 __device__ inline double sph_cubic_kernel(double q)
@@ -110,27 +122,52 @@ __device__ inline int wrap_pixel(int z, int nbins)
 }
 
 // Per-pair constants of LineAbsorption::add_tau_particle and SingleAbsorber.
+// Each wavefront keeps the constants of its current pair in an LDS slot: the
+// values are uniform across the wave, and reading them with broadcast LDS
+// loads keeps them out of the vector registers of the per-pixel loop.
 struct PairTau {
+    double cv[kNGrid - 1];   // kernel sample velocities / b
     double vel;      // velfac*ppos + pvel (absorption.cpp:234)
     double b;        // btherm (absorption.cpp:220)
     double inv_b;
-    double ampdens;  // amp * dens (absorption.cpp:244,255)
     double dvk;      // 2 vhigh / NGRID (singleabs.h:146)
     double vhigh, vdr2, vsmooth, aa;
-    double cv[kNGrid - 1];   // kernel sample velocities / b
+    double scale;    // amp * dens * dvk / velfac: taulast = scale * (sample sum)
+    // Far wing: for a pixel centred at U (units of b) with |U| >= ufar the
+    // 7 x np samples sum to Y (C0 + C1 Y + ... + C9 Y^9), Y = (ufar/U)^2.
+    double ufar;
+    float ufar2;
     float W[kNGrid - 1];     // kernel weights K(q_i)
+    float Cw[kWingTerms];
     VoigtA va;
     int zmax;
-    bool fp64;
+    int fp64;
+    int np;          // sub-samples per pixel (1 = midpoint)
+    int wing_ready;  // Cw computed
 };
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline double shfl_d(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __shfl((int)(b & 0xffffffffll), src, 64);
+    const int hi = __shfl((int)(b >> 32), src, 64);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // sum_i K(q_i) Re w((vv_i - vouter)/b + i a), FP32 profile
 __device__ inline double inner32(const PairTau &P, double vouter, const float *gtab)
 {
     const double s = vouter * P.inv_b;
+    const VoigtA va = P.va;
     float acc = 0.0f;
-#pragma unroll
-    for (int i = 0; i < kNGrid - 1; ++i) acc = __builtin_fmaf(P.W[i], voigt32(P.va, P.cv[i] - s, gtab), acc);
+#pragma unroll 1
+    for (int i = 0; i < kNGrid - 1; ++i) acc = __builtin_fmaf(P.W[i], voigt32(va, P.cv[i] - s, gtab), acc);
     return (double)acc;
 }
 
@@ -161,27 +198,104 @@ __device__ inline double taulast_pixel(const PairTau &P, double vlow, double bin
 {
     const double vhighp = vlow + bintov;
     const double width = vhighp - vlow;
-    double outer;
+    if (!k64) {
+        const double U = (vlow + 0.5 * bintov) * P.inv_b;
+        if (fabs(U) >= P.ufar) {
+            const float Y = P.ufar2 * __builtin_amdgcn_rcpf((float)(U * U));
+            float S = P.Cw[kWingTerms - 1];
+#pragma unroll
+            for (int t = kWingTerms - 2; t >= 0; --t) S = __builtin_fmaf(S, Y, P.Cw[t]);
+            return P.scale * (double)(S * Y);
+        }
+    }
     if (width < P.b / 2.) {
         const double mid = (vhighp + vlow) / 2.;
-        outer = P.dvk * (k64 ? inner64(P, mid, kernel) : inner32(P, mid, gtab));
-    } else {
-        const int np = 2 * ceil(width / (P.b / 2.) / 2) + 1.;
-        const double dv = width / (np - 1);
-        double total = 0.5 * (k64 ? inner64(P, vlow, kernel) : inner32(P, vlow, gtab));
-        for (int i = 1; i < np - 1; ++i) {
-            const double vv = i * dv + vlow;
-            total += k64 ? inner64(P, vv, kernel) : inner32(P, vv, gtab);
-        }
-        total += 0.5 * (k64 ? inner64(P, vhighp, kernel) : inner32(P, vhighp, gtab));
-        outer = P.dvk * total / (np - 1);
+        return P.scale * (k64 ? inner64(P, mid, kernel) : inner32(P, mid, gtab));
     }
-    return P.ampdens * outer / velfac;
+    const int np = 2 * ceil(width / (P.b / 2.) / 2) + 1.;
+    const double dv = width / (np - 1);
+    double total = 0.5 * (k64 ? inner64(P, vlow, kernel) : inner32(P, vlow, gtab));
+    for (int i = 1; i < np - 1; ++i) {
+        const double vv = i * dv + vlow;
+        total += k64 ? inner64(P, vv, kernel) : inner32(P, vv, gtab);
+    }
+    total += 0.5 * (k64 ? inner64(P, vhighp, kernel) : inner32(P, vhighp, gtab));
+    return P.scale * (total / (np - 1));
 }
 
-// Set up one pair; returns false if the particle does not reach the sightline.
+// Far-wing coefficients of one pair.  The sample offsets from the pixel
+// centre are c = x_i - y_j with x_i = vv_i/b the kernel samples (weights W_i)
+// and y_j the sub-sample offsets of tau_kern_outer (singleabs.h:110-125,
+// trapezoid weights t_j); both sets are symmetric, so the even moments of c
+// factor:  m_n = sum_k C(2n,2k) X_k Y_(n-k),  X_k = sum W_i x_i^2k,
+// Y_k = sum t_j y_j^2k.  Offsets are measured in units of ufar so that all
+// coefficients stay within FP32 range:  C'_n = sum_(k+j=n) g_k ufar^(-2k-2)
+// C(2n+1,2j) m'_j with m'_j the moments of c/ufar.  Lane n < 10 builds the
+// n-th moments and coefficient and stores C'_n.
+__device__ inline void wing_setup(PairTau &P, double bintov, int lane)
+{
+    const double inv_u = 1.0 / P.ufar;
+    const int np = P.np;
+    const int n = lane < kWingTerms ? lane : 0;
+    // X_n: kernel samples
+    double Xn = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < kNGrid - 1; ++i) {
+        const double x2 = (P.cv[i] * inv_u) * (P.cv[i] * inv_u);
+        double pw = 1.0;
+        for (int r = 0; r < n; ++r) pw *= x2;
+        Xn += (double)P.W[i] * pw;
+    }
+    // Y_n: pixel sub-samples
+    double Yn = 1.0;
+    if (np > 1) {
+        const double dv = bintov / (np - 1);
+        Yn = 0.0;
+        for (int j = 0; j < np; ++j) {
+            const double y = (j * dv - 0.5 * bintov) * P.inv_b * inv_u;
+            const double t = ((j == 0 || j == np - 1) ? 0.5 : 1.0) / (np - 1);
+            double pw = 1.0;
+            for (int r = 0; r < n; ++r) pw *= y * y;
+            Yn += t * pw;
+        }
+    } else if (n > 0) {
+        Yn = 0.0;
+    }
+    // m'_n = sum_k C(2n,2k) X_k Y_(n-k)
+    double m = 0.0, bin = 1.0;
+#pragma unroll 1
+    for (int k = 0; k < kWingTerms; ++k) {
+        const double xk = shfl_d(Xn, k);
+        const double yk = shfl_d(Yn, n - k < 0 ? 0 : n - k);
+        if (k > 0) bin = bin * (double)(2 * n - 2 * k + 2) * (double)(2 * n - 2 * k + 1) /
+                         ((double)(2 * k - 1) * (double)(2 * k));
+        if (k <= n) m += bin * xk * yk;
+    }
+    // C'_n = sum_(k+j=n) g_k ufar^(-2k-2) C(2n+1, 2j) m'_j
+    const double a = P.aa, a2 = a * a, asq = a / kSqrtPi, iu2 = inv_u * inv_u;
+    double C = 0.0, binom = 1.0, gs = asq * iu2;
+    // walk j = 0..n (k = n..0): start from k = n, so scale ufar^(-2n-2)
+    for (int r = 0; r < n; ++r) gs *= iu2;
+#pragma unroll 1
+    for (int j = 0; j < kWingTerms; ++j) {
+        const double mj = shfl_d(m, j);
+        if (j > 0) {
+            binom = binom * (double)(2 * n + 3 - 2 * j) * (double)(2 * n + 2 - 2 * j) /
+                    ((double)(2 * j - 1) * (double)(2 * j));
+            gs *= P.ufar * P.ufar;
+        }
+        if (j <= n) {
+            const int k = n - j;
+            C += gs * (c_wing_p[k] - (k > 0 ? a2 * c_wing_q[k - 1] : 0.0)) * binom * mj;
+        }
+    }
+    if (lane < kWingTerms) P.Cw[lane] = (float)C;
+}
+
+// Set up one pair in the wave's LDS slot; returns false if the particle does
+// not reach the sightline.  Every lane of the wave calls it.
 __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const LosInfo &los, int p,
-                                  PairTau &P)
+                                  PairTau &P, int lane)
 {
     const float *pp = &A.pos[3 * (size_t)p];
     const double dr2 = pair_dr2(los, pp, A.box);
@@ -190,37 +304,67 @@ __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const
     const float dens = L.dens[p];
     const float ppos = pp[los.ax - 1];
     const float pvel = A.vel[3 * (size_t)p + los.ax - 1];
-    P.b = L.bfac * sqrt((double)temp);
+    const double b = L.bfac * sqrt((double)temp);
     const float sm2 = smooth * smooth;
     if ((double)sm2 - dr2 <= 0) return false;            // absorption.cpp:232
-    P.vel = A.velfac * (double)ppos + (double)pvel;
+    const double vel = A.velfac * (double)ppos + (double)pvel;
     double val1 = A.velfac * dr2;
     val1 *= A.velfac;
-    P.vdr2 = val1;
-    P.vsmooth = A.velfac * (double)smooth;
-    P.aa = L.voigt_fac / P.b;
-    P.vhigh = P.vsmooth * P.vsmooth > P.vdr2 ? sqrt(P.vsmooth * P.vsmooth - P.vdr2) : 0;   // singleabs.h:83
-    P.inv_b = 1.0 / P.b;
+    const double vdr2 = val1;
+    const double vsmooth = A.velfac * (double)smooth;
+    const double aa = L.voigt_fac / b;
+    const double vhigh = vsmooth * vsmooth > vdr2 ? sqrt(vsmooth * vsmooth - vdr2) : 0;   // singleabs.h:83
+    const double inv_b = 1.0 / b;
     const double bintov = A.vbox / A.nbins;
-    const double amp = L.sigma_a / sqrt(kPi) * (kLight / 1e5 / P.b);
-    P.ampdens = amp * (double)dens;
-    P.zmax = (int)floor(P.vel / bintov);
-    P.dvk = 2. * P.vhigh / kNGrid;
-#pragma unroll
-    for (int i = 1; i < kNGrid; ++i) {
-        const double vv = i * P.dvk - P.vhigh;
-        const double q = sqrt(P.vdr2 + vv * vv) / P.vsmooth;
-        P.cv[i - 1] = vv * P.inv_b;
-        P.W[i - 1] = (float)kernel_weight(A.kernel, q);
+    const double amp = L.sigma_a / sqrt(kPi) * (kLight / 1e5 / b);
+    const double dvk = 2. * vhigh / kNGrid;
+    const int fp64 = !(aa <= (double)kA32Max);
+    int np = 1;
+    if (!(bintov < b / 2.)) np = 2 * ceil(bintov / (b / 2.) / 2) + 1.;
+    const double dmax = (vhigh - dvk) * inv_b + (np > 1 ? 0.5 * bintov * inv_b : 0.0);
+    const double ufar = fmax(7.0 + dmax, 3.0 * dmax);
+    wave_sync();   // the previous pair's constants are no longer read
+    if (lane < kNGrid - 1) {
+        const double vv = (lane + 1) * dvk - vhigh;
+        const double q = sqrt(vdr2 + vv * vv) / vsmooth;
+        P.cv[lane] = vv * inv_b;
+        P.W[lane] = (float)kernel_weight(A.kernel, q);
     }
-    P.fp64 = !(P.aa <= (double)kA32Max);
-    P.va = make_voigt_a(P.fp64 ? 0.0 : P.aa);
+    if (lane == 0) {
+        P.vel = vel;
+        P.b = b;
+        P.inv_b = inv_b;
+        P.dvk = dvk;
+        P.vhigh = vhigh;
+        P.vdr2 = vdr2;
+        P.vsmooth = vsmooth;
+        P.aa = aa;
+        P.zmax = (int)floor(vel / bintov);
+        P.fp64 = fp64;
+        P.va = make_voigt_a(fp64 ? 0.0 : aa);
+        P.scale = amp * (double)dens * dvk / A.velfac;
+        P.np = np;
+        P.ufar = ufar;
+        P.ufar2 = (float)(ufar * ufar);
+        P.wing_ready = 0;
+    }
+    wave_sync();
     return true;
 }
 
 struct LaneStats {
-    long long pairs = 0, pixels = 0, px64 = 0, pairs64 = 0;
+    long long pairs = 0, pixels = 0, px64 = 0, pairs64 = 0, samples = 0, evaluated = 0;
 };
+
+// Voigt samples the reference evaluates for one pixel (singleabs.h:110-125).
+__device__ inline int pixel_samples(const PairTau &P, double vlow, double bintov)
+{
+    const double vhighp = vlow + bintov;
+    const double width = vhighp - vlow;
+    if (width < P.b / 2.) return kNGrid - 1;
+    const int np = 2 * ceil(width / (P.b / 2.) / 2) + 1.;
+    return (kNGrid - 1) * np;
+}
 
 template <bool kLds>
 __device__ inline void row_add(double *p, double v)
@@ -231,33 +375,90 @@ __device__ inline void row_add(double *p, double v)
         unsafeAtomicAdd(p, v);    // global_atomic_add_f64
 }
 
-// Walk one direction: pixels z = z0, z0 + dir, ... for at most `count` pixels.
-template <bool k64, bool kLds>
-__device__ inline void walk(const DepositArgs &A, const PairTau &P, double *row, const float *gtab, int z0,
-                            int dir, int count, int lane, LaneStats &st)
+// Walk modes: the FP32 deposit kernel defers any decision it cannot make
+// exactly; the fix-up kernel re-evaluates guard-band pixels in FP64, or
+// evaluates whole pairs in FP64.
+enum { kWalkMain = 0, kWalkGuarded = 1, kWalkFull64 = 2 };
+
+__device__ inline void push_fixup(const DepositArgs &A, long long k, int item, int z0, int count, int dir)
+{
+    const int slot = atomicAdd(&A.fix_count[0], 1);
+    if (slot < A.fix_cap)
+        A.fix[slot] = Fixup{k, item, z0, count, dir};
+    else
+        atomicOr(&A.fix_count[1], 1);
+}
+
+// One chunk of 64 consecutive pixels of a walk; returns true when the walk
+// is over (tail cut reached, or the decision deferred to the fix-up kernel).
+template <int kMode, bool kLds>
+__device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double *row, const float *gtab, int z0,
+                                  int dir, int count, int c, int lane, LaneStats &st, long long k, int item)
 {
     const double bintov = A.vbox / A.nbins;
-    for (int c = 0; c < count; c += 64) {
-        const int idx = c + lane;
-        const bool valid = idx < count;
-        const int z = z0 + dir * idx;
-        double t = 0;
-        if (valid) {
-            const double vlow = z * bintov - P.vel;                 // absorption.cpp:252
-            t = taulast_pixel<k64>(P, vlow, bintov, A.velfac, A.kernel, gtab);
-            if (!k64 && fabs(t - A.tautail) <= kGuard * A.tautail) {
+    const double tt = A.tautail;
+    const int idx = c + lane;
+    const bool valid = idx < count;
+    const int z = z0 + dir * idx;
+    double t = 0, vlow = 0;
+    bool guard = false;
+    if (valid) {
+        vlow = z * bintov - P.vel;                                  // absorption.cpp:252
+        t = taulast_pixel<kMode == kWalkFull64>(P, vlow, bintov, A.velfac, A.kernel, gtab);
+        st.evaluated++;
+        if (kMode != kWalkFull64 && fabs(t - tt) <= kGuard * tt) {
+            if (kMode == kWalkGuarded) {
                 t = taulast_pixel<true>(P, vlow, bintov, A.velfac, A.kernel, gtab);
                 st.px64++;
+            } else {
+                guard = true;
             }
         }
-        const unsigned long long below = __ballot(valid && t < A.tautail);
-        const int last = below ? __builtin_ctzll(below) : 63;
-        if (valid && lane <= last) {
-            row_add<kLds>(&row[wrap_pixel(z, A.nbins)], t);
-            st.pixels++;
-        }
-        if (below) break;
     }
+    const unsigned long long mb = __ballot(valid && !guard && t < tt);
+    const unsigned long long mg = __ballot(guard);
+    const unsigned long long mall = mb | mg;
+    const int first = mall ? __builtin_ctzll(mall) : 64;
+    if (valid && (lane < first || (lane == first && ((mb >> lane) & 1ull)))) {
+        row_add<kLds>(&row[wrap_pixel(z, A.nbins)], t);
+        st.pixels++;
+        if (A.stats) st.samples += pixel_samples(P, vlow, bintov);
+    }
+    if (kMode == kWalkMain && first < 64 && ((mg >> first) & 1ull)) {
+        // the FP32 value of the deciding pixel is too close to tautail
+        if (lane == 0) push_fixup(A, k, item, z0 + dir * (c + first), count - (c + first), dir);
+        return true;
+    }
+    return mb != 0 || c + 64 >= count;
+}
+
+// Walk one direction: pixels z = z0, z0 + dir, ... for at most `count` pixels,
+// depositing until (and including) the first pixel with taulast < tautail.
+// The far-wing coefficients are built the first time a chunk reaches a pixel
+// that uses them, between the near and the far phase of the walk.
+template <int kMode, bool kLds>
+__device__ inline void walk(const DepositArgs &A, PairTau &P, double *row, const float *gtab, int z0, int dir,
+                            int count, int lane, LaneStats &st, long long k, int item)
+{
+    const double bintov = A.vbox / A.nbins;
+    int c = 0;
+    if (kMode != kWalkFull64 && !P.wing_ready) {
+        for (; c < count; c += 64) {
+            const int idx = min(c + 63, count - 1);
+            const double vend = (z0 + dir * idx) * bintov - P.vel + 0.5 * bintov;
+            // |U| grows along the walk away from the particle; the chunk's far
+            // end decides whether any of its pixels needs the expansion
+            const double U0 = ((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
+            if (fmax(fabs(vend * P.inv_b), fabs(U0)) >= P.ufar) break;
+            if (walk_chunk<kMode, kLds>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item)) return;
+        }
+        if (c >= count) return;
+        wing_setup(P, bintov, lane);
+        if (lane == 0) P.wing_ready = 1;
+        wave_sync();
+    }
+    for (; c < count; c += 64)
+        if (walk_chunk<kMode, kLds>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item)) return;
 }
 
 template <bool kLds>
@@ -266,9 +467,11 @@ __global__ __launch_bounds__(kThreads) void tau_kernel(DepositArgs A, const Work
 {
     extern __shared__ double smem[];
     __shared__ float gtab[kGTableSize];
+    __shared__ PairTau pslot[kWaves];
     __shared__ int s_item;
     for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    PairTau &P = pslot[wave];
     LaneStats st;
     int it = use_queue ? 0 : blockIdx.x;
     for (;;) {
@@ -279,11 +482,13 @@ __global__ __launch_bounds__(kThreads) void tau_kernel(DepositArgs A, const Work
             __syncthreads();
             item = s_item;
         } else {
+            __syncthreads();
             item = it;
             it += gridDim.x;
         }
         if (item >= A.nitems) break;
-        const WorkItem w = items[item_base + item];
+        const int gitem = item_base + item;
+        const WorkItem w = items[gitem];
         const LineDesc L = line_of(A, w.line);
         double *out_row = L.out + (size_t)w.los * A.nbins;
         double *row = kLds ? smem : out_row;
@@ -292,19 +497,18 @@ __global__ __launch_bounds__(kThreads) void tau_kernel(DepositArgs A, const Work
             __syncthreads();
         }
         const LosInfo los = los_info(A, w.los);
+        const int half = A.nbins / 2;
         for (long long k = w.p0 + wave; k < w.p1; k += kWaves) {
-            PairTau P;
-            if (!setup_pair(A, L, los, A.pair_part[k], P)) continue;
+            if (!setup_pair(A, L, los, A.pair_part[k], P, lane)) continue;
             st.pairs++;
-            const int half = A.nbins / 2;
+            const int zmax = P.zmax;
             if (P.fp64) {
                 st.pairs64++;
-                walk<true, kLds>(A, P, row, gtab, P.zmax, 1, half, lane, st);
-                walk<true, kLds>(A, P, row, gtab, P.zmax - 1, -1, half, lane, st);
-            } else {
-                walk<false, kLds>(A, P, row, gtab, P.zmax, 1, half, lane, st);
-                walk<false, kLds>(A, P, row, gtab, P.zmax - 1, -1, half, lane, st);
+                if (lane == 0) push_fixup(A, k, gitem, zmax, half, 0);
+                continue;
             }
+            walk<kWalkMain, kLds>(A, P, row, gtab, zmax, 1, half, lane, st, k, gitem);
+            walk<kWalkMain, kLds>(A, P, row, gtab, zmax - 1, -1, half, lane, st, k, gitem);
         }
         if (kLds) {
             __syncthreads();
@@ -321,11 +525,47 @@ __global__ __launch_bounds__(kThreads) void tau_kernel(DepositArgs A, const Work
     if (A.stats) {
         // lanes count pixels individually; pair counts are per wave (lane 0)
         atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.pixels);
-        atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st.px64);
+        atomicAdd((unsigned long long *)&A.stats[4], (unsigned long long)st.samples);
+        atomicAdd((unsigned long long *)&A.stats[5], (unsigned long long)st.evaluated);
         if (lane == 0) {
             atomicAdd((unsigned long long *)&A.stats[0], (unsigned long long)st.pairs);
             atomicAdd((unsigned long long *)&A.stats[3], (unsigned long long)st.pairs64);
         }
+    }
+}
+
+// Deferred walks, after the deposit kernel has written its rows: one wave per
+// fix-up entry, deposits straight into the output with FP64 atomics.
+__global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, const WorkItem *items)
+{
+    __shared__ float gtab[kGTableSize];
+    __shared__ PairTau pslot[kWaves];
+    for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    PairTau &P = pslot[wave];
+    LaneStats st;
+    const int n = min(A.fix_count[0], A.fix_cap);
+    const int half = A.nbins / 2;
+    for (int e = blockIdx.x * kWaves + wave; e < n; e += gridDim.x * kWaves) {
+        const Fixup f = A.fix[e];
+        const WorkItem w = items[f.item];
+        const LineDesc L = line_of(A, w.line);
+        double *out_row = L.out + (size_t)w.los * A.nbins;
+        const LosInfo los = los_info(A, w.los);
+        if (!setup_pair(A, L, los, A.pair_part[f.k], P, lane)) continue;
+        if (f.dir == 0) {
+            walk<kWalkFull64, false>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k, f.item);
+            walk<kWalkFull64, false>(A, P, out_row, gtab, P.zmax - 1, -1, half, lane, st, f.k, f.item);
+        } else {
+            walk<kWalkGuarded, false>(A, P, out_row, gtab, f.z0, f.dir, f.count, lane, st, f.k, f.item);
+        }
+    }
+    if (A.stats) {
+        atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.pixels);
+        atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st.px64);
+        atomicAdd((unsigned long long *)&A.stats[4], (unsigned long long)st.samples);
+        atomicAdd((unsigned long long *)&A.stats[5], (unsigned long long)st.evaluated);
     }
 }
 
@@ -412,6 +652,32 @@ __global__ __launch_bounds__(kThreads) void colden_kernel(DepositArgs A, const W
 }
 
 // ---------------------------------------------------------------------------
+// Isolated throughput of the FP32 sample evaluator (kernel weight x profile),
+// no memory traffic: each thread walks u = (tid + i * stride) * du over
+// [0, umax), the mix of core and wing samples of a pixel window.
+
+__global__ __launch_bounds__(kThreads) void voigt_peak_kernel(long long per_thread, double du, double umax,
+                                                              float a, float *sink)
+{
+    __shared__ float gtab[kGTableSize];
+    for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
+    __syncthreads();
+    const VoigtA va = make_voigt_a(a);
+    const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long nthreads = (long long)gridDim.x * blockDim.x;
+    float acc = 0.0f;
+    const float w = 0.25f;
+    double u = fmod((double)tid * du, umax);
+    const double step = fmod((double)nthreads * du, umax);
+    for (long long i = 0; i < per_thread; ++i) {
+        acc = __builtin_fmaf(w, voigt32(va, u, gtab), acc);
+        u += step;
+        if (u >= umax) u -= umax;
+    }
+    if (acc == -1.0f) sink[0] = acc;   // keeps the loop live; never true
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 
 size_t deposit_lds_bytes(int nbins) { return sizeof(double) * (size_t)nbins; }
@@ -437,6 +703,21 @@ int launch_deposit(bool tau, bool lds, const DepositArgs &A, const WorkItem *ite
             colden_kernel<false><<<nblocks, kThreads, 0, s>>>(A, items, item_base, use_queue);
         }
     }
+    FSAMD_HIP(hipGetLastError());
+    return FSAMD_OK;
+}
+
+int launch_tau_fixup(const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s)
+{
+    tau_fixup_kernel<<<nblocks, kThreads, 0, s>>>(A, items);
+    FSAMD_HIP(hipGetLastError());
+    return FSAMD_OK;
+}
+
+int launch_voigt_peak(long long per_thread, int nblocks, double du, double umax, float a, float *sink,
+                      hipStream_t s)
+{
+    voigt_peak_kernel<<<nblocks, kThreads, 0, s>>>(per_thread, du, umax, a, sink);
     FSAMD_HIP(hipGetLastError());
     return FSAMD_OK;
 }

@@ -66,6 +66,18 @@ struct LineDesc {
 
 constexpr int kMaxLines = 8;   // lines per deposit launch
 
+// A walk handed from the FP32 deposit kernel to the FP64-capable fix-up
+// kernel: pixels z0, z0 + dir, ... (count of them) of pair k in work item
+// `item`; dir == 0 means the whole pair (both directions) in FP64.
+struct Fixup {
+    long long k;
+    int item;
+    int z0;
+    int count;
+    int dir;
+};
+constexpr int kNStats = 6;
+
 struct DepositArgs {
     int nbins;
     int kernel;
@@ -85,7 +97,10 @@ struct DepositArgs {
     int nlines;
     int nitems;
     int *queue;                 // work-queue counter (zeroed before launch)
-    long long *stats;           // optional per-launch counters (pairs, pixels, fp64 pixels)
+    Fixup *fix;                 // fix-up list
+    int *fix_count;             // [0] entries appended, [1] overflow flag
+    int fix_cap;
+    long long *stats;           // optional counters, kNStats entries (see fsamd_last_stats)
 };
 
 // Host-side launchers (return FSAMD_OK or an error code).

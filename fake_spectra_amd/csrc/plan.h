@@ -1,7 +1,9 @@
 // The binning plan: device neighbour lists and the buffers that own them.
 #pragma once
 #include "fsamd_internal.h"
+#include "deposit.h"
 
+#include <memory>
 #include <vector>
 
 namespace fsamd {
@@ -50,4 +52,27 @@ struct fsamd_plan {
     fsamd::DevBuf order;                  // sightlines by descending pair count
     std::vector<int> order_host;
     std::vector<long long> off_host;
+    // Deposit launch state, reused across calls so that a call allocates,
+    // copies and synchronizes nothing once warm (and can be graph-captured).
+    struct ItemSet {
+        int K;
+        long long chunk;
+        bool split;
+        fsamd::DevBuf buf;
+        std::vector<fsamd::WorkItem> host;
+    };
+    std::vector<std::unique_ptr<ItemSet>> item_sets;
+    fsamd::DevBuf queue;        // work-queue counters, one per launch group
+    fsamd::DevBuf lines_dev;    // LineDesc table
+    std::vector<fsamd::LineDesc> lines_host;
+    fsamd::DevBuf stats;        // kNStats counters
+    fsamd::DevBuf fix;          // fix-up list (FP64 walks)
+    fsamd::DevBuf fix_count;    // [count, overflow] per line group
+    int fix_cap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~fsamd_plan()
+    {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
 };

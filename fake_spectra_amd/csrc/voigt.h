@@ -140,12 +140,13 @@ __device__ inline float voigt32(const VoigtA &v, double u64, const float *gtab)
 // ---------------------------------------------------------------------------
 // FP64 Faddeeva w(z), y >= 0.
 
-__device__ inline double dawson64(double x)
+__device__ __noinline__ double dawson64(double x)
 {
     const double ax = fabs(x);
     if (ax < 0.2) {
         double t = 1.0, s = 1.0;
         const double m = -2.0 * ax * ax;
+#pragma unroll 1
         for (int k = 1; k < 14; k++) {
             t *= m / (2 * k + 1);
             s += t;
@@ -157,6 +158,7 @@ __device__ inline double dawson64(double x)
     const double xp = ax - n0 * h;
     const double e1 = exp(2.0 * xp * h), e2 = e1 * e1;
     double up = e1, dn = 1.0 / e1, s = 0.0;
+#pragma unroll 1
     for (int i = 0; i < 22; i++) {
         const double n = 2 * i + 1;
         const double g = exp(-(n * h) * (n * h));
@@ -168,7 +170,7 @@ __device__ inline double dawson64(double x)
     return x < 0 ? -d : d;
 }
 
-__device__ inline void w64_contfrac(double x, double y, int nterms, double &wr, double &wi)
+__device__ __noinline__ void w64_contfrac(double x, double y, int nterms, double &wr, double &wi)
 {
     double tr = x, ti = y;
     for (int k = nterms; k >= 1; k--) {
@@ -182,7 +184,7 @@ __device__ inline void w64_contfrac(double x, double y, int nterms, double &wr, 
     wi = tr / den / kSqrtPi;
 }
 
-__device__ inline void w64_taylor_y(double x, double y, double &wr, double &wi)
+__device__ __noinline__ void w64_taylor_y(double x, double y, double &wr, double &wi)
 {
     const double w0r = exp(-x * x), w0i = kTwoOverSqrtPi * dawson64(x);
     const double w1r = -2.0 * x * w0r, w1i = -2.0 * x * w0i + kTwoOverSqrtPi;

@@ -44,7 +44,7 @@ class _DeviceArray:
 
     def __init__(self, ptr: int, shape, typestr: str):
         self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr,
-                                         "data": (int(ptr), True), "version": 3, "strides": None}
+                                         "data": (int(ptr), False), "version": 3, "strides": None}
 
 
 def _require(t: torch.Tensor, dtype: torch.dtype, name: str) -> torch.Tensor:
@@ -181,9 +181,10 @@ def colden(plan: Plan, nbins: int, kernel: int, velfac: float, dens: Sequence[to
 
 
 def last_stats():
-    buf = (C.c_longlong * 4)()
+    buf = (C.c_longlong * 6)()
     N.check(N.lib().fsamd_last_stats(buf))
-    return {"pairs": buf[0], "pixels": buf[1], "fp64_pixels": buf[2], "fp64_pairs": buf[3]}
+    return {"pairs": buf[0], "pixels": buf[1], "fp64_pixels": buf[2], "fp64_pairs": buf[3],
+            "voigt_samples": buf[4], "evaluated_pixels": buf[5], "gpu_ms": float(N.lib().fsamd_last_ms())}
 
 
 def set_stats(enable: bool):

@@ -70,14 +70,14 @@ typedef struct fsamd_line {
  * part_int.cpp:20-51, with LineAbsorption::add_tau_particle,
  * absorption.cpp:212-279).  kernel: 0 tophat, 1 cubic spline.
  * mode: FSAMD_MODE_BATCHED or FSAMD_MODE_PLAIN. */
-int fsamd_tau(const fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail,
+int fsamd_tau(fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail,
               const float *pos, const float *vel, const float *temp, const float *h,
               const fsamd_line *lines, int nlines, int mode, void *stream);
 
 /* Column density (ParticleInterp::compute_colden, part_int.cpp:53-84, with
  * LineAbsorption::add_colden_particle, absorption.cpp:167-210).  Only dens
  * and out of each line are used. */
-int fsamd_colden(const fsamd_plan *plan, int nbins, int kernel, double velfac, const float *pos,
+int fsamd_colden(fsamd_plan *plan, int nbins, int kernel, double velfac, const float *pos,
                  const float *h, const fsamd_line *lines, int nlines, void *stream);
 
 /* Drop-in of _Particle_Interpolate (py_module.cpp:103-233) on device
@@ -91,11 +91,22 @@ int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double bo
                                const float *h, long long npart, const int *axis,
                                const double *cofm, int nlos, double *out, void *stream);
 
-/* Counters of the last fsamd_tau call on this thread (host): pairs deposited,
- * pixels deposited, pixels re-evaluated in FP64 at the tail cut, pairs
- * evaluated entirely in FP64.  Filled only when fsamd_set_stats(1). */
+/* Counters of the last fsamd_tau call on this thread (host), six entries:
+ * pairs deposited, pixels deposited, pixels re-evaluated in FP64 at the tail
+ * cut, pairs evaluated entirely in FP64, Voigt samples of the deposited
+ * pixels (7 x sub-samples, as the reference counts them), pixels evaluated
+ * (deposited or not); and the GPU time of its launches in ms (hipEvents on
+ * the call's stream).  Collected only after fsamd_set_stats(1), which makes
+ * fsamd_tau synchronize the stream. */
 int fsamd_set_stats(int enable);
-int fsamd_last_stats(long long *out4);
+int fsamd_last_stats(long long *out6);
+double fsamd_last_ms(void);
+
+/* Diagnostic with no counterpart in the reference: isolated throughput of the
+ * FP32 sample evaluator (kernel weight x Voigt profile) in samples/s, over
+ * arguments u swept uniformly in [0, umax) with damping parameter a, no
+ * memory traffic.  The denominator of the deposit's VALU fraction. */
+int fsamd_voigt_peak(double umax, double a, double *samples_per_s, void *stream);
 
 #ifdef __cplusplus
 }
