@@ -10,7 +10,7 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libfsamd.so")
+LIB_PATH = os.environ.get("FSAMD_LIB") or os.path.join(HERE, "_lib", "libfsamd.so")
 
 FSAMD_OK = 0
 FSAMD_ERR_ARG = 1
@@ -24,7 +24,7 @@ MODE_PLAIN = 1
 EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan_destroy",
            "fsamd_plan_npairs", "fsamd_plan_pairs", "fsamd_plan_near_lines", "fsamd_tau",
            "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats", "fsamd_last_ms",
-           "fsamd_voigt_peak")
+           "fsamd_voigt_peak", "fsamd_set_group_size")
 
 
 class FsamdLine(C.Structure):
@@ -67,6 +67,8 @@ def _declare(L):
     L.fsamd_last_ms.restype = d
     L.fsamd_voigt_peak.argtypes = [d, d, C.POINTER(d), vp]
     L.fsamd_voigt_peak.restype = i
+    L.fsamd_set_group_size.argtypes = [i]
+    L.fsamd_set_group_size.restype = i
 
 
 def lib():
@@ -80,6 +82,11 @@ def lib():
             _declare(L)
             _lib = L
     return _lib
+
+
+def set_group_size(g: int):
+    """Pixels per lane group of the tau kernel: 16, 32 or 64."""
+    check(lib().fsamd_set_group_size(int(g)))
 
 
 def check(rc: int):

@@ -14,6 +14,7 @@ static thread_local std::string g_error;
 static thread_local long long g_stats[kNStats] = {0, 0, 0, 0, 0, 0};
 static thread_local int g_stats_on = 0;
 static thread_local double g_last_ms = 0;
+static int g_group_size = 16;   // pixels per lane group of the tau kernel (16, 32 or 64)
 
 void set_error(const std::string &msg) { g_error = msg; }
 
@@ -147,6 +148,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         FSAMD_HIP(hipEventRecord(P->ev[0], s));
     }
     const bool plain = (mode == FSAMD_MODE_PLAIN);
+    const int gsize = g_group_size;
     if (tau) {
         // room for deferred FP64 walks: a few percent of pair-directions
         const long long want = std::max<long long>(65536, 2 * P->npairs * std::min(nlines, kMaxLines) / 32);
@@ -192,9 +194,9 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
             if (!plain) {
                 A.nitems = (int)items.size();
                 if ((rc = launch_fill_i32(A.queue, 0, 1, s))) return rc;
-                const int per_cu = deposit_blocks_per_cu(tau, lds, nbins);
+                const int per_cu = deposit_blocks_per_cu(tau, lds, gsize, nbins);
                 const int nblocks = (int)std::min<long long>((long long)per_cu * dev.ncu, A.nitems);
-                if ((rc = launch_deposit(tau, lds, A, set->buf.as<WorkItem>(), 0, nblocks, 1, s))) return rc;
+                if ((rc = launch_deposit(tau, lds, gsize, A, set->buf.as<WorkItem>(), 0, nblocks, 1, s))) return rc;
             } else {
                 // one launch per (sightline, line), in the plan's item order
                 size_t i = 0;
@@ -202,7 +204,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
                     size_t j = i;
                     while (j < items.size() && items[j].los == items[i].los && items[j].line == items[i].line) ++j;
                     A.nitems = (int)(j - i);
-                    if ((rc = launch_deposit(tau, lds, A, set->buf.as<WorkItem>(), (int)i, A.nitems, 0, s)))
+                    if ((rc = launch_deposit(tau, lds, gsize, A, set->buf.as<WorkItem>(), (int)i, A.nitems, 0, s)))
                         return rc;
                     i = j;
                 }
@@ -338,6 +340,13 @@ int fsamd_last_stats(long long *out6)
 }
 
 double fsamd_last_ms(void) { return g_last_ms; }
+
+int fsamd_set_group_size(int g)
+{
+    if (g != 16 && g != 32 && g != 64) return fail(FSAMD_ERR_ARG, "group size must be 16, 32 or 64");
+    g_group_size = g;
+    return FSAMD_OK;
+}
 
 int fsamd_voigt_peak(double umax, double a, double *samples_per_s, void *stream)
 {

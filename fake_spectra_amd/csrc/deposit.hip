@@ -152,11 +152,13 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+// Shuffle of a double from lane `src` of the same kG-lane group.
+template <int kG>
 __device__ inline double shfl_d(double v, int src)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __shfl((int)(b & 0xffffffffll), src, 64);
-    const int hi = __shfl((int)(b >> 32), src, 64);
+    const int lo = __shfl((int)(b & 0xffffffffll), src, kG);
+    const int hi = __shfl((int)(b >> 32), src, kG);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -232,6 +234,7 @@ __device__ inline double taulast_pixel(const PairTau &P, double vlow, double bin
 // coefficients stay within FP32 range:  C'_n = sum_(k+j=n) g_k ufar^(-2k-2)
 // C(2n+1,2j) m'_j with m'_j the moments of c/ufar.  Lane n < 10 builds the
 // n-th moments and coefficient and stores C'_n.
+template <int kG>
 __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
 {
     const double inv_u = 1.0 / P.ufar;
@@ -265,8 +268,8 @@ __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
     double m = 0.0, bin = 1.0;
 #pragma unroll 1
     for (int k = 0; k < kWingTerms; ++k) {
-        const double xk = shfl_d(Xn, k);
-        const double yk = shfl_d(Yn, n - k < 0 ? 0 : n - k);
+        const double xk = shfl_d<kG>(Xn, k);
+        const double yk = shfl_d<kG>(Yn, n - k < 0 ? 0 : n - k);
         if (k > 0) bin = bin * (double)(2 * n - 2 * k + 2) * (double)(2 * n - 2 * k + 1) /
                          ((double)(2 * k - 1) * (double)(2 * k));
         if (k <= n) m += bin * xk * yk;
@@ -278,7 +281,7 @@ __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
     for (int r = 0; r < n; ++r) gs *= iu2;
 #pragma unroll 1
     for (int j = 0; j < kWingTerms; ++j) {
-        const double mj = shfl_d(m, j);
+        const double mj = shfl_d<kG>(m, j);
         if (j > 0) {
             binom = binom * (double)(2 * n + 3 - 2 * j) * (double)(2 * n + 2 - 2 * j) /
                     ((double)(2 * j - 1) * (double)(2 * j));
@@ -294,6 +297,7 @@ __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
 
 // Set up one pair in the wave's LDS slot; returns false if the particle does
 // not reach the sightline.  Every lane of the wave calls it.
+template <int kG>
 __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const LosInfo &los, int p,
                                   PairTau &P, int lane)
 {
@@ -389,16 +393,28 @@ __device__ inline void push_fixup(const DepositArgs &A, long long k, int item, i
         atomicOr(&A.fix_count[1], 1);
 }
 
-// One chunk of 64 consecutive pixels of a walk; returns true when the walk
-// is over (tail cut reached, or the decision deferred to the fix-up kernel).
-template <int kMode, bool kLds>
-__device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double *row, const float *gtab, int z0,
-                                  int dir, int count, int c, int lane, LaneStats &st, long long k, int item)
+// Lanes of a wave are split into groups of kG lanes; each group walks its own
+// (pair, direction) stream, kG consecutive pixels per step.
+template <int kG>
+__device__ inline unsigned long long group_bits(unsigned long long m, int lane)
 {
+    if (kG == 64) return m;
+    return (m >> ((lane / kG) * kG)) & ((1ull << kG) - 1ull);
+}
+
+// One step of kG consecutive pixels of a walk (lane glane of the group);
+// returns true when the walk is over (tail cut reached, end of the window, or
+// the decision deferred to the fix-up kernel).
+template <int kG, int kMode, bool kLds>
+__device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double *row, const float *gtab, int z0,
+                                  int dir, int count, int c, int lane, LaneStats &st, long long k, int item,
+                                  bool active)
+{
+    const int glane = lane % kG;
     const double bintov = A.vbox / A.nbins;
     const double tt = A.tautail;
-    const int idx = c + lane;
-    const bool valid = idx < count;
+    const int idx = c + glane;
+    const bool valid = active && idx < count;
     const int z = z0 + dir * idx;
     double t = 0, vlow = 0;
     bool guard = false;
@@ -415,74 +431,78 @@ __device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double
             }
         }
     }
-    const unsigned long long mb = __ballot(valid && !guard && t < tt);
-    const unsigned long long mg = __ballot(guard);
+    const unsigned long long mb = group_bits<kG>(__ballot(valid && !guard && t < tt), lane);
+    const unsigned long long mg = group_bits<kG>(__ballot(guard), lane);
     const unsigned long long mall = mb | mg;
-    const int first = mall ? __builtin_ctzll(mall) : 64;
-    if (valid && (lane < first || (lane == first && ((mb >> lane) & 1ull)))) {
+    const int first = mall ? __builtin_ctzll(mall) : kG;
+    if (valid && (glane < first || (glane == first && ((mb >> glane) & 1ull)))) {
         row_add<kLds>(&row[wrap_pixel(z, A.nbins)], t);
         st.pixels++;
         if (A.stats) st.samples += pixel_samples(P, vlow, bintov);
     }
-    if (kMode == kWalkMain && first < 64 && ((mg >> first) & 1ull)) {
+    if (kMode == kWalkMain && first < kG && ((mg >> first) & 1ull)) {
         // the FP32 value of the deciding pixel is too close to tautail
-        if (lane == 0) push_fixup(A, k, item, z0 + dir * (c + first), count - (c + first), dir);
+        if (active && glane == 0) push_fixup(A, k, item, z0 + dir * (c + first), count - (c + first), dir);
         return true;
     }
-    return mb != 0 || c + 64 >= count;
+    return mb != 0 || c + kG >= count;
 }
 
-// Walk one direction: pixels z = z0, z0 + dir, ... for at most `count` pixels,
-// depositing until (and including) the first pixel with taulast < tautail.
-// The far-wing coefficients are built the first time a chunk reaches a pixel
-// that uses them, between the near and the far phase of the walk.
-template <int kMode, bool kLds>
-__device__ inline void walk(const DepositArgs &A, PairTau &P, double *row, const float *gtab, int z0, int dir,
-                            int count, int lane, LaneStats &st, long long k, int item)
+// Whether the next step of a walk reaches pixels that use the far-wing
+// expansion (|U| only grows along a walk after its first pixel).
+__device__ inline bool step_needs_wing(const PairTau &P, double bintov, int z0, int dir, int c, int kg, int count)
+{
+    const int last = min(c + kg - 1, count - 1);
+    const double U0 = ((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
+    const double U1 = ((z0 + dir * last) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
+    return fmax(fabs(U0), fabs(U1)) >= P.ufar;
+}
+
+// Walk one direction with the whole wave (fix-up kernel): pixels z0, z0+dir,
+// ... for at most `count` pixels, until and including the tail cut.
+template <int kMode>
+__device__ inline void walk64(const DepositArgs &A, PairTau &P, double *row, const float *gtab, int z0, int dir,
+                              int count, int lane, LaneStats &st, long long k, int item)
 {
     const double bintov = A.vbox / A.nbins;
-    int c = 0;
-    if (kMode != kWalkFull64 && !P.wing_ready) {
-        for (; c < count; c += 64) {
-            const int idx = min(c + 63, count - 1);
-            const double vend = (z0 + dir * idx) * bintov - P.vel + 0.5 * bintov;
-            // |U| grows along the walk away from the particle; the chunk's far
-            // end decides whether any of its pixels needs the expansion
-            const double U0 = ((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
-            if (fmax(fabs(vend * P.inv_b), fabs(U0)) >= P.ufar) break;
-            if (walk_chunk<kMode, kLds>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item)) return;
+    for (int c = 0; c < count; c += 64) {
+        if (kMode != kWalkFull64 && !P.wing_ready && step_needs_wing(P, bintov, z0, dir, c, 64, count)) {
+            wing_setup<64>(P, bintov, lane);
+            if (lane == 0) P.wing_ready = 1;
+            wave_sync();
         }
-        if (c >= count) return;
-        wing_setup(P, bintov, lane);
-        if (lane == 0) P.wing_ready = 1;
-        wave_sync();
+        if (walk_chunk<64, kMode, false>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item, true)) return;
     }
-    for (; c < count; c += 64)
-        if (walk_chunk<kMode, kLds>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item)) return;
 }
 
-template <bool kLds>
+template <int kG, bool kLds>
 __global__ __launch_bounds__(kThreads) void tau_kernel(DepositArgs A, const WorkItem *items, int item_base,
                                                        int use_queue)
 {
+    constexpr int kGroups = kThreads / kG;
     extern __shared__ double smem[];
     __shared__ float gtab[kGTableSize];
-    __shared__ PairTau pslot[kWaves];
+    __shared__ PairTau pslot[kGroups];
     __shared__ int s_item;
+    __shared__ unsigned long long s_next;
     for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    PairTau &P = pslot[wave];
+    const int lane = threadIdx.x & 63;
+    const int glane = lane % kG;
+    const int gid = threadIdx.x / kG;
+    const int gleader = (lane / kG) * kG;
+    PairTau &P = pslot[gid];
+    const double bintov = A.vbox / A.nbins;
+    const int half = A.nbins / 2;
     LaneStats st;
     int it = use_queue ? 0 : blockIdx.x;
     for (;;) {
         int item;
+        __syncthreads();
         if (use_queue) {
-            __syncthreads();
             if (threadIdx.x == 0) s_item = atomicAdd(A.queue, 1);
             __syncthreads();
             item = s_item;
         } else {
-            __syncthreads();
             item = it;
             it += gridDim.x;
         }
@@ -492,23 +512,54 @@ __global__ __launch_bounds__(kThreads) void tau_kernel(DepositArgs A, const Work
         const LineDesc L = line_of(A, w.line);
         double *out_row = L.out + (size_t)w.los * A.nbins;
         double *row = kLds ? smem : out_row;
-        if (kLds) {
+        if (kLds)
             for (int j = threadIdx.x; j < A.nbins; j += kThreads) row[j] = 0.0;
-            __syncthreads();
-        }
+        if (threadIdx.x == 0) s_next = (unsigned long long)w.p0;
+        __syncthreads();
         const LosInfo los = los_info(A, w.los);
-        const int half = A.nbins / 2;
-        for (long long k = w.p0 + wave; k < w.p1; k += kWaves) {
-            if (!setup_pair(A, L, los, A.pair_part[k], P, lane)) continue;
-            st.pairs++;
-            const int zmax = P.zmax;
-            if (P.fp64) {
-                st.pairs64++;
-                if (lane == 0) push_fixup(A, k, gitem, zmax, half, 0);
-                continue;
+        // group state: dir 0 = needs a pair, +1/-1 = walking, 2 = no pairs left
+        long long k = 0;
+        int dir = 0, c = 0, z0 = 0;
+        for (;;) {
+            if (dir == 0) {
+                unsigned long long kk = 0;
+                if (glane == 0) kk = atomicAdd(&s_next, 1ull);
+                kk = (unsigned long long)__shfl((long long)kk, gleader, 64);
+                if ((long long)kk >= w.p1) {
+                    dir = 2;
+                } else if (setup_pair<kG>(A, L, los, A.pair_part[kk], P, glane)) {
+                    k = (long long)kk;
+                    st.pairs += (glane == 0);
+                    if (P.fp64) {
+                        st.pairs64 += (glane == 0);
+                        if (glane == 0) push_fixup(A, k, gitem, P.zmax, half, 0);
+                    } else {
+                        dir = 1;
+                        z0 = P.zmax;
+                        c = 0;
+                    }
+                }
             }
-            walk<kWalkMain, kLds>(A, P, row, gtab, zmax, 1, half, lane, st, k, gitem);
-            walk<kWalkMain, kLds>(A, P, row, gtab, zmax - 1, -1, half, lane, st, k, gitem);
+            const bool walking = (dir == 1 || dir == -1);
+            if (!__ballot(dir != 2)) break;
+            if (walking && !P.wing_ready && step_needs_wing(P, bintov, z0, dir, c, kG, half)) {
+                wing_setup<kG>(P, bintov, glane);
+                if (glane == 0) P.wing_ready = 1;
+                wave_sync();
+            }
+            const bool done = walk_chunk<kG, kWalkMain, kLds>(A, P, row, gtab, z0, dir, half, c, lane, st, k, gitem,
+                                                              walking);
+            if (walking) {
+                if (!done) {
+                    c += kG;
+                } else if (dir == 1) {
+                    dir = -1;
+                    z0 = P.zmax - 1;
+                    c = 0;
+                } else {
+                    dir = 0;
+                }
+            }
         }
         if (kLds) {
             __syncthreads();
@@ -523,14 +574,11 @@ __global__ __launch_bounds__(kThreads) void tau_kernel(DepositArgs A, const Work
         }
     }
     if (A.stats) {
-        // lanes count pixels individually; pair counts are per wave (lane 0)
+        atomicAdd((unsigned long long *)&A.stats[0], (unsigned long long)st.pairs);
         atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.pixels);
+        atomicAdd((unsigned long long *)&A.stats[3], (unsigned long long)st.pairs64);
         atomicAdd((unsigned long long *)&A.stats[4], (unsigned long long)st.samples);
         atomicAdd((unsigned long long *)&A.stats[5], (unsigned long long)st.evaluated);
-        if (lane == 0) {
-            atomicAdd((unsigned long long *)&A.stats[0], (unsigned long long)st.pairs);
-            atomicAdd((unsigned long long *)&A.stats[3], (unsigned long long)st.pairs64);
-        }
     }
 }
 
@@ -553,12 +601,12 @@ __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, cons
         const LineDesc L = line_of(A, w.line);
         double *out_row = L.out + (size_t)w.los * A.nbins;
         const LosInfo los = los_info(A, w.los);
-        if (!setup_pair(A, L, los, A.pair_part[f.k], P, lane)) continue;
+        if (!setup_pair<64>(A, L, los, A.pair_part[f.k], P, lane)) continue;
         if (f.dir == 0) {
-            walk<kWalkFull64, false>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k, f.item);
-            walk<kWalkFull64, false>(A, P, out_row, gtab, P.zmax - 1, -1, half, lane, st, f.k, f.item);
+            walk64<kWalkFull64>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k, f.item);
+            walk64<kWalkFull64>(A, P, out_row, gtab, P.zmax - 1, -1, half, lane, st, f.k, f.item);
         } else {
-            walk<kWalkGuarded, false>(A, P, out_row, gtab, f.z0, f.dir, f.count, lane, st, f.k, f.item);
+            walk64<kWalkGuarded>(A, P, out_row, gtab, f.z0, f.dir, f.count, lane, st, f.k, f.item);
         }
     }
     if (A.stats) {
@@ -682,18 +730,33 @@ __global__ __launch_bounds__(kThreads) void voigt_peak_kernel(long long per_thre
 
 size_t deposit_lds_bytes(int nbins) { return sizeof(double) * (size_t)nbins; }
 
-int launch_deposit(bool tau, bool lds, const DepositArgs &A, const WorkItem *items, int item_base, int nblocks,
-                   int use_queue, hipStream_t s)
+template <int kG, bool kLds>
+static int launch_tau(const DepositArgs &A, const WorkItem *items, int item_base, int nblocks, int use_queue,
+                      size_t shm, hipStream_t s)
+{
+    if (kLds)
+        FSAMD_HIP(hipFuncSetAttribute((const void *)tau_kernel<kG, kLds>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)shm));
+    tau_kernel<kG, kLds><<<nblocks, kThreads, shm, s>>>(A, items, item_base, use_queue);
+    return FSAMD_OK;
+}
+
+int launch_deposit(bool tau, bool lds, int gsize, const DepositArgs &A, const WorkItem *items, int item_base,
+                   int nblocks, int use_queue, hipStream_t s)
 {
     const size_t shm = lds ? deposit_lds_bytes(A.nbins) : 0;
+    int rc = FSAMD_OK;
     if (tau) {
-        if (lds) {
-            FSAMD_HIP(hipFuncSetAttribute((const void *)tau_kernel<true>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-            tau_kernel<true><<<nblocks, kThreads, shm, s>>>(A, items, item_base, use_queue);
-        } else {
-            tau_kernel<false><<<nblocks, kThreads, 0, s>>>(A, items, item_base, use_queue);
-        }
+        if (gsize == 16)
+            rc = lds ? launch_tau<16, true>(A, items, item_base, nblocks, use_queue, shm, s)
+                     : launch_tau<16, false>(A, items, item_base, nblocks, use_queue, shm, s);
+        else if (gsize == 32)
+            rc = lds ? launch_tau<32, true>(A, items, item_base, nblocks, use_queue, shm, s)
+                     : launch_tau<32, false>(A, items, item_base, nblocks, use_queue, shm, s);
+        else
+            rc = lds ? launch_tau<64, true>(A, items, item_base, nblocks, use_queue, shm, s)
+                     : launch_tau<64, false>(A, items, item_base, nblocks, use_queue, shm, s);
+        if (rc) return rc;
     } else {
         if (lds) {
             FSAMD_HIP(hipFuncSetAttribute((const void *)colden_kernel<true>,
@@ -722,22 +785,34 @@ int launch_voigt_peak(long long per_thread, int nblocks, double du, double umax,
     return FSAMD_OK;
 }
 
-int deposit_blocks_per_cu(bool tau, bool lds, int nbins)
+template <int kG>
+static int occ_tau(bool lds, size_t shm)
+{
+    int nb = 0;
+    hipError_t e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, true>, kThreads, shm)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, false>, kThreads, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return nb;
+}
+
+int deposit_blocks_per_cu(bool tau, bool lds, int gsize, int nbins)
 {
     int nb = 0;
     const size_t shm = lds ? deposit_lds_bytes(nbins) : 0;
-    hipError_t e;
-    if (tau)
-        e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<true>, kThreads, shm)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<false>, kThreads, 0);
-    else
-        e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, colden_kernel<true>, kThreads, shm)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, colden_kernel<false>, kThreads, 0);
-    if (e != hipSuccess || nb < 1) {
-        (void)hipGetLastError();
-        nb = 1;
+    if (tau) {
+        nb = gsize == 16 ? occ_tau<16>(lds, shm) : gsize == 32 ? occ_tau<32>(lds, shm) : occ_tau<64>(lds, shm);
+    } else {
+        hipError_t e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, colden_kernel<true>, kThreads, shm)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, colden_kernel<false>, kThreads, 0);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            nb = 1;
+        }
     }
-    return nb;
+    return nb < 1 ? 1 : nb;
 }
 
 }  // namespace fsamd

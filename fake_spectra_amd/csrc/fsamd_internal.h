@@ -4,7 +4,7 @@
 #include <cstdint>
 #include <string>
 
-#include "../../include/fake_spectra_amd.h"
+#include "fake_spectra_amd.h"
 
 namespace fsamd {
 

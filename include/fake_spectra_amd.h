@@ -102,6 +102,11 @@ int fsamd_set_stats(int enable);
 int fsamd_last_stats(long long *out6);
 double fsamd_last_ms(void);
 
+/* Tuning knob with no counterpart in the reference: pixels walked per lane
+ * group of the tau kernel (16, 32 or 64; a 64-lane wavefront walks 64/g
+ * (pair, direction) streams at once).  Process-wide; default 16. */
+int fsamd_set_group_size(int g);
+
 /* Diagnostic with no counterpart in the reference: isolated throughput of the
  * FP32 sample evaluator (kernel weight x Voigt profile) in samples/s, over
  * arguments u swept uniformly in [0, umax) with damping parameter a, no

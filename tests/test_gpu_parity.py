@@ -104,6 +104,22 @@ def test_tau_configs_vs_oracle(oracle, idx, scale, nlos):
     assert_colden_close(col.cpu().numpy(), oracle.interpolate(*w.interp_args(None, False)), "c%d colden" % idx)
 
 
+@pytest.mark.parametrize("g", [16, 32, 64])
+def test_group_sizes_match_oracle(oracle, g):
+    """Every lane-group width of the tau kernel gives the same spectra."""
+    from fake_spectra_amd import _native as N
+    w = synth.config(1, scale=1 / 64, nlos=8)
+    plan = plan_for(w)
+    N.set_group_size(g)
+    try:
+        outs = SP.tau(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), w.lines,
+                      [dev(w.dens_for(ln)) for ln in w.lines], w.tautail)
+    finally:
+        N.set_group_size(16)
+    for ln, o in zip(w.lines, outs):
+        assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)), "G=%d %s" % (g, ln.name))
+
+
 def test_plain_mode_equals_batched():
     w = synth.config(2, scale=1 / 64, nlos=4)
     plan = plan_for(w)
