@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-adjoint", action="store_true")
     return ap.parse_args()
 
 
@@ -67,6 +68,54 @@ def cpu_baseline(w, budget_s):
             "sample": "%s line %s, %d of %d sightlines (every %d-th), all %d particles, index table + deposit, "
                       "%.1f s" % ("reference C++ -O2 -ffast-math OpenMP" if kind == "reference" else "C restatement",
                                   line.name, len(sel), w.nlos, k, w.npart, t)}
+
+
+def adjoint_leg(steps):
+    """Forward and forward+adjoint on the differentiable workload (128^3
+    particles, 1024 sightlines x 1024 pixels, H I 1215): loss
+    sum (exp(-tau) - F)^2, F from the same field with T scaled by 1.1."""
+    import numpy as np
+    import torch
+    from fake_spectra_amd import synth, spectra_priv as SP
+    w = synth.config(3)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    pos, hh, ax, cofm = dev(w.pos), dev(w.hh), dev(w.axis), dev(w.cofm)
+    vel, temp, dens = dev(w.vel), dev(w.temp), dev(w.dens["H1"])
+    line = w.lines[0]
+    plan = SP.Plan(w.box, pos, hh, ax, cofm)
+    F = torch.exp(-SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, dev(w.temp * np.float32(1.1)), [line], [dens])[0])
+    plan.close()
+    out = [torch.empty((w.nlos, w.nbins), dtype=torch.float64, device="cuda")]
+
+    def fwd():
+        pl = SP.Plan(w.box, pos, hh, ax, cofm)
+        SP.tau(pl, w.nbins, w.kernel, w.velfac, vel, temp, [line], [dens], w.tautail, out=out)
+        return pl
+
+    def fwd_adj():
+        pl = SP.Plan(w.box, pos, hh, ax, cofm)
+        pl.keep_windows(True)
+        t = SP.tau(pl, w.nbins, w.kernel, w.velfac, vel, temp, [line], [dens], w.tautail, out=out)[0]
+        e = torch.exp(-t)
+        g = (-2.0 * (e - F) * e).contiguous()      # dL/dtau
+        SP.tau_adjoint(pl, w.nbins, w.kernel, w.velfac, vel, temp, [line], [dens], [g])
+        return pl
+
+    res = {}
+    for name, fn in (("forward", fwd), ("forward_adjoint", fwd_adj)):
+        fn().close()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn().close()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / steps
+    units = w.nlos * w.nbins
+    return {"workload": "128^3 particles, 1024 sightlines x 1024 pixels, H I 1215, loss sum (exp(-tau)-F)^2",
+            "forward_ms": 1e3 * res["forward"], "forward_adjoint_ms": 1e3 * res["forward_adjoint"],
+            "forward_value": units / res["forward"], "forward_adjoint_value": units / res["forward_adjoint"],
+            "adjoint_over_forward": res["forward_adjoint"] / res["forward"], "unit": "sightline-pixels/s",
+            "note": "binning included in both; fwd+adjoint records windows and runs the adjoint kernel"}
 
 
 def main():
@@ -180,6 +229,8 @@ def main():
                         "batched_over_plain": tplain / tbatch,
                         "note": "deposit only, same binning; one launch per sightline and line"}
         plan.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(max(2, args.steps))
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_seconds)
         out["cpu_baseline"] = cb

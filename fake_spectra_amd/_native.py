@@ -24,12 +24,18 @@ MODE_PLAIN = 1
 EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan_destroy",
            "fsamd_plan_npairs", "fsamd_plan_pairs", "fsamd_plan_near_lines", "fsamd_tau",
            "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats", "fsamd_last_ms",
-           "fsamd_voigt_peak", "fsamd_set_group_size")
+           "fsamd_voigt_peak", "fsamd_set_group_size", "fsamd_plan_keep_windows", "fsamd_tau_adjoint")
 
 
 class FsamdLine(C.Structure):
     _fields_ = [("lambda_cm", C.c_double), ("gamma", C.c_double), ("fosc", C.c_double),
                 ("amumass", C.c_double), ("dens", C.c_void_p), ("out", C.c_void_p)]
+
+
+class FsamdLineGrad(C.Structure):
+    _fields_ = [("lambda_cm", C.c_double), ("gamma", C.c_double), ("fosc", C.c_double),
+                ("amumass", C.c_double), ("dens", C.c_void_p), ("grad_tau", C.c_void_p),
+                ("grad_dens", C.c_void_p)]
 
 
 _lock = threading.Lock()
@@ -69,6 +75,10 @@ def _declare(L):
     L.fsamd_voigt_peak.restype = i
     L.fsamd_set_group_size.argtypes = [i]
     L.fsamd_set_group_size.restype = i
+    L.fsamd_plan_keep_windows.argtypes = [vp, i]
+    L.fsamd_plan_keep_windows.restype = i
+    L.fsamd_tau_adjoint.argtypes = [vp, i, i, d, vp, vp, vp, vp, C.POINTER(FsamdLineGrad), i, vp, vp, vp]
+    L.fsamd_tau_adjoint.restype = i
 
 
 def lib():

@@ -149,6 +149,18 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
     }
     const bool plain = (mode == FSAMD_MODE_PLAIN);
     const int gsize = g_group_size;
+    const bool rec = tau && P->keep_windows;
+    if (rec) {
+        const size_t wbytes = sizeof(int) * 2 * (size_t)std::max<long long>(P->npairs, 1) * nlines;
+        if (P->win.bytes < wbytes) {
+            FSAMD_HIP(hipStreamSynchronize(s));
+            if ((rc = P->win.alloc(wbytes))) return rc;
+        }
+        if ((rc = launch_fill_i32(P->win.as<int>(), 0, (long long)(wbytes / sizeof(int)), s))) return rc;
+        P->win_lines = nlines;
+        P->win_nbins = nbins;
+        P->win_kernel = kernel;
+    }
     if (tau) {
         // room for deferred FP64 walks: a few percent of pair-directions
         const long long want = std::max<long long>(65536, 2 * P->npairs * std::min(nlines, kMaxLines) / 32);
@@ -178,6 +190,8 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         A.lines = P->lines_dev.as<LineDesc>() + k0;
         A.queue = P->queue.as<int>() + g;
         A.stats = g_stats_on ? P->stats.as<long long>() : nullptr;
+        A.npairs = P->npairs;
+        A.win = rec ? P->win.as<int>() + (size_t)k0 * 2 * P->npairs : nullptr;
         fsamd_plan::ItemSet *set = nullptr;
         if ((rc = item_set(P, K, plain ? kPlainChunk : (1ll << 62), &set))) return rc;
         const std::vector<WorkItem> &items = set->host;
@@ -324,6 +338,81 @@ int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double bo
     (void)hipStreamSynchronize((hipStream_t)stream);
     fsamd_plan_destroy(plan);
     return rc;
+}
+
+int fsamd_plan_keep_windows(fsamd_plan *plan, int enable)
+{
+    if (!plan) return fail(FSAMD_ERR_ARG, "plan is NULL");
+    plan->keep_windows = enable != 0;
+    if (!enable) {
+        plan->win.release();
+        plan->win_lines = 0;
+    }
+    return FSAMD_OK;
+}
+
+int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const float *pos, const float *vel,
+                      const float *temp, const float *h, const fsamd_line_grad *lines, int nlines, double *grad_temp,
+                      double *grad_vel, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!P) return fail(FSAMD_ERR_ARG, "plan is NULL");
+    if (!lines || nlines < 1 || !grad_temp || !grad_vel) return fail(FSAMD_ERR_ARG, "NULL argument");
+    if (!P->keep_windows || P->win_lines != nlines || P->win_nbins != nbins || P->win_kernel != kernel)
+        return fail(FSAMD_ERR_ARG, "fsamd_tau_adjoint needs the windows of a matching fsamd_tau call on this plan "
+                                   "(fsamd_plan_keep_windows(plan, 1) before the forward)");
+    static thread_local DeviceInfo dev;
+    int rc;
+    if (dev.ncu == 0 && (rc = device_info(dev))) return rc;
+    if (deposit_lds_bytes(nbins) + 2048 > (size_t)dev.max_lds)
+        return fail(FSAMD_ERR_UNSUPPORTED, "adjoint rows must fit in LDS (nbins too large)");
+    std::vector<LineDesc> ld(nlines);
+    for (int i = 0; i < nlines; ++i) {
+        if (!lines[i].dens || !lines[i].grad_tau || !lines[i].grad_dens)
+            return fail(FSAMD_ERR_ARG, "line without dens, grad_tau or grad_dens");
+        fsamd_line l{lines[i].lambda_cm, lines[i].gamma, lines[i].fosc, lines[i].amumass, lines[i].dens,
+                     const_cast<double *>(lines[i].grad_tau)};
+        line_consts(l, ld[i]);
+        ld[i].grad_dens = lines[i].grad_dens;
+    }
+    if ((rc = upload_lines(P, ld, s))) return rc;
+    const int ngroups = (nlines + kMaxLines - 1) / kMaxLines;
+    if (P->queue.bytes < sizeof(int) * (size_t)ngroups && (rc = P->queue.alloc(sizeof(int) * ngroups))) return rc;
+    const int gsize = g_group_size;
+    for (int g = 0; g < ngroups; ++g) {
+        const int k0 = g * kMaxLines;
+        const int K = std::min(kMaxLines, nlines - k0);
+        DepositArgs A;
+        std::memset(&A, 0, sizeof(A));
+        A.nbins = nbins;
+        A.kernel = kernel;
+        A.velfac = velfac;
+        A.vbox = P->box * velfac;
+        A.box = P->box;
+        A.pos = pos;
+        A.vel = vel;
+        A.temp = temp;
+        A.h = h;
+        A.axis = P->axis.as<int>();
+        A.cofm = P->cofm.as<double>();
+        A.line_off = P->line_off.as<long long>();
+        A.pair_part = P->pair_part.as<int>();
+        A.nlines = K;
+        A.lines = P->lines_dev.as<LineDesc>() + k0;
+        A.queue = P->queue.as<int>() + g;
+        A.npairs = P->npairs;
+        A.win = P->win.as<int>() + (size_t)k0 * 2 * P->npairs;
+        A.grad_temp = grad_temp;
+        A.grad_vel = grad_vel;
+        fsamd_plan::ItemSet *set = nullptr;
+        if ((rc = item_set(P, K, 1ll << 62, &set))) return rc;
+        if (set->host.empty()) continue;
+        A.nitems = (int)set->host.size();
+        if ((rc = launch_fill_i32(A.queue, 0, 1, s))) return rc;
+        const int nblocks = (int)std::min<long long>((long long)adjoint_blocks_per_cu(gsize, nbins) * dev.ncu, A.nitems);
+        if ((rc = launch_tau_adjoint(gsize, A, set->buf.as<WorkItem>(), nblocks, s))) return rc;
+    }
+    return FSAMD_OK;
 }
 
 int fsamd_set_stats(int enable)

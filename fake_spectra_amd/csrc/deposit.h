@@ -20,6 +20,8 @@ size_t deposit_lds_bytes(int nbins);
 int launch_deposit(bool tau, bool lds, int gsize, const DepositArgs &A, const WorkItem *items, int item_base,
                    int nblocks, int use_queue, hipStream_t s);
 int deposit_blocks_per_cu(bool tau, bool lds, int gsize, int nbins);
+int launch_tau_adjoint(int gsize, const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);
+int adjoint_blocks_per_cu(int gsize, int nbins);
 int launch_tau_fixup(const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);
 int launch_voigt_peak(long long per_thread, int nblocks, double du, double umax, float a, float *sink,
                       hipStream_t s);

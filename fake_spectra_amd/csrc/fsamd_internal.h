@@ -61,7 +61,8 @@ struct LineDesc {
     double bfac;       // absorption.cpp:155
     double voigt_fac;  // absorption.cpp:156
     const float *dens; // [N] species density
-    double *out;       // [nlos, nbins]
+    double *out;       // [nlos, nbins]; in the adjoint: dL/dtau (read only)
+    double *grad_dens; // adjoint: [N] +=
 };
 
 constexpr int kMaxLines = 8;   // lines per deposit launch
@@ -97,6 +98,11 @@ struct DepositArgs {
     int nlines;
     int nitems;
     int *queue;                 // work-queue counter (zeroed before launch)
+    int *win;                   // optional [nlines][npairs][2] pixels walked per direction
+    long long npairs;
+    // adjoint only
+    double *grad_temp;          // [N] +=
+    double *grad_vel;           // [N,3] +=
     Fixup *fix;                 // fix-up list
     int *fix_count;             // [0] entries appended, [1] overflow flag
     int fix_cap;

@@ -69,6 +69,11 @@ struct fsamd_plan {
     fsamd::DevBuf fix;          // fix-up list (FP64 walks)
     fsamd::DevBuf fix_count;    // [count, overflow] per line group
     int fix_cap = 0;
+    // pixels walked per (line, pair, direction) by the last fsamd_tau call,
+    // kept for the adjoint when keep_windows is set
+    bool keep_windows = false;
+    fsamd::DevBuf win;
+    int win_lines = 0, win_nbins = 0, win_kernel = -1;
     hipEvent_t ev[2] = {nullptr, nullptr};
     ~fsamd_plan()
     {

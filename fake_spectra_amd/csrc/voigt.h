@@ -138,6 +138,78 @@ __device__ inline float voigt32(const VoigtA &v, double u64, const float *gtab)
 }
 
 // ---------------------------------------------------------------------------
+// Profile derivatives for the adjoint, FP32.  With z = u + ia, w' = -2zw + 2i/sqrt(pi):
+//   dH/du = Re w' = -2(uH - aL),   dH/da = -Im w' = 2(uL + aH) - 2/sqrt(pi),
+// L = Im w.  The adjoint needs H, Hu = dH/du and F = u dH/du + a dH/da
+// (the response to the thermal width b, all sample offsets scaling as 1/b):
+//   F = -2(u^2 - a^2) H + 4 a u L - 2a/sqrt(pi).
+// Core |u| < 7: L = -exp(a^2-u^2) sin(2au) + (2/sqrt(pi)) [D(u) - (a^2/2) D''(u)],
+// D = (1 - G)/(2u), D'' = -2D - 2uG.  Wing |u| >= 7: the asymptotic series
+// H = (a/sqrt(pi)) sum p_k u^(-2k-2) gives Hu and F termwise, without the
+// cancellation of the closed forms (a^2-relative corrections dropped there).
+struct VoigtGrad {
+    float H, Hu, F;
+};
+
+__device__ inline VoigtGrad voigt32_grad(const VoigtA &v, double u64, const float *gtab)
+{
+    const float sgn = u64 < 0 ? -1.0f : 1.0f;
+    const double au = u64 < 0 ? -u64 : u64;
+    const float uh = (float)au;
+    VoigtGrad r;
+    if (uh >= kVoigtCoreMax) {
+        const float x = __builtin_amdgcn_rcpf(uh * uh);
+        const float P[9] = FSAMD_WING_P;
+        const float Q[3] = FSAMD_WING_Q;
+        float s = P[8], d = 18.0f * P[8], f = -17.0f * P[8];
+#pragma unroll
+        for (int k = 7; k >= 0; --k) {
+            s = __builtin_fmaf(s, x, P[k]);
+            d = __builtin_fmaf(d, x, (2.0f * k + 2.0f) * P[k]);
+            f = __builtin_fmaf(f, x, -(2.0f * k + 1.0f) * P[k]);
+        }
+        float q = __builtin_fmaf(Q[2], x, Q[1]);
+        q = __builtin_fmaf(q, x, Q[0]);
+        r.H = v.a_sqpi * x * __builtin_fmaf(-v.a2 * x, q, s);
+        r.Hu = -sgn * v.a_sqpi * x * d * __builtin_amdgcn_rcpf(uh);
+        r.F = v.a_sqpi * x * f;
+        return r;
+    }
+    const float ul = (float)(au - (double)uh);
+    const float p = uh * uh;
+    const float dd = __builtin_fmaf(2.0f * uh, ul, __builtin_fmaf(uh, uh, -p));
+    const float E = exp_neg_df(p, dd) * v.ea2;          // exp(a^2 - u^2)
+    const float c = 2.0f * v.a * uh;
+    const float c2 = c * c;
+    float cs = __builtin_fmaf(c2, -1.0f / 40320.0f, 1.0f / 720.0f);
+    cs = __builtin_fmaf(c2, -cs, 1.0f / 24.0f);
+    cs = __builtin_fmaf(c2, -cs, 0.5f);
+    cs = __builtin_fmaf(c2, -cs, 1.0f);
+    float sn = __builtin_fmaf(c2, -1.0f / 5040.0f, 1.0f / 120.0f);
+    sn = __builtin_fmaf(c2, sn, -1.0f / 6.0f);
+    sn = c * __builtin_fmaf(c2, sn, 1.0f);
+    const float G = g_of_u(uh, gtab);
+    float D;
+    if (uh < 0.25f) {
+        // D(u) = u - 2u^3/3 + 4u^5/15 - 8u^7/105 + 16u^9/945
+        float t = __builtin_fmaf(p, 16.0f / 945.0f, -8.0f / 105.0f);
+        t = __builtin_fmaf(p, t, 4.0f / 15.0f);
+        t = __builtin_fmaf(p, t, -2.0f / 3.0f);
+        D = uh * __builtin_fmaf(p, t, 1.0f);
+    } else {
+        D = (1.0f - G) * 0.5f * __builtin_amdgcn_rcpf(uh);
+    }
+    const float D2 = -2.0f * D - 2.0f * uh * G;
+    const float D3 = __builtin_fmaf(__builtin_fmaf(4.0f, p, -6.0f), G, 2.0f);
+    const float H = __builtin_fmaf(-v.two_a_sqpi, __builtin_fmaf(-v.a2_6, D3, G), E * cs);
+    const float L = __builtin_fmaf(kTwoOverSqrtPiF, __builtin_fmaf(-0.5f * v.a2, D2, D), -E * sn);
+    r.H = H;
+    r.Hu = sgn * (-2.0f * __builtin_fmaf(uh, H, -v.a * L));
+    r.F = __builtin_fmaf(-2.0f * (p - v.a2), H, __builtin_fmaf(4.0f * v.a * uh, L, -v.two_a_sqpi));
+    return r;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 Faddeeva w(z), y >= 0.
 
 __device__ __noinline__ double dawson64(double x)

@@ -116,6 +116,11 @@ class Plan:
         p = torch.as_tensor(_DeviceArray(parts.value, (m,), "<i4"), device=self.pos.device).clone()
         return o, p
 
+    def keep_windows(self, enable: bool = True):
+        """Record the deposited window of every (line, pair, direction) in the
+        next tau() call, as the adjoint needs."""
+        N.check(N.lib().fsamd_plan_keep_windows(self.handle, 1 if enable else 0))
+
     def near_lines(self) -> torch.Tensor:
         """Sorted ids of particles near any sightline (_near_lines)."""
         out = torch.empty(max(self.npart, 1), dtype=torch.int32, device=self.pos.device)
@@ -162,6 +167,31 @@ def tau(plan: Plan, nbins: int, kernel: int, velfac: float, vel: torch.Tensor, t
     N.check(N.lib().fsamd_tau(plan.handle, int(nbins), int(kernel), float(velfac), float(tautail), _ptr(plan.pos),
                               _ptr(vel), _ptr(temp), _ptr(plan.h), arr, len(quads), int(mode), _stream()))
     return list(out)
+
+
+def tau_adjoint(plan: Plan, nbins: int, kernel: int, velfac: float, vel: torch.Tensor, temp: torch.Tensor,
+                lines: Sequence, dens: Sequence[torch.Tensor], grad_tau: Sequence[torch.Tensor]):
+    """dL/d dens (one float64 [N] per line), dL/dT (float64 [N]) and dL/d vel
+    (float64 [N,3]) from dL/dtau of each line, through the last tau() call on
+    this plan (made with plan.keep_windows(True))."""
+    vel = _require(vel, torch.float32, "vel")
+    temp = _require(temp, torch.float32, "temp")
+    quads = [_as_quad(x) for x in lines]
+    dens = [_require(d, torch.float32, "dens") for d in dens]
+    gts = [_require(g, torch.float64, "grad_tau") for g in grad_tau]
+    for g in gts:
+        if g.shape != (plan.nlos, nbins):
+            raise ValueError("grad_tau must have shape (nlos, nbins)")
+    dv = plan.pos.device
+    gd = [torch.zeros(plan.npart, dtype=torch.float64, device=dv) for _ in quads]
+    gT = torch.zeros(plan.npart, dtype=torch.float64, device=dv)
+    gv = torch.zeros((plan.npart, 3), dtype=torch.float64, device=dv)
+    arr = (N.FsamdLineGrad * len(quads))()
+    for i, ((lam, gam, fosc, amu), d, g, o) in enumerate(zip(quads, dens, gts, gd)):
+        arr[i] = N.FsamdLineGrad(lam, gam, fosc, amu, d.data_ptr(), g.data_ptr(), o.data_ptr())
+    N.check(N.lib().fsamd_tau_adjoint(plan.handle, int(nbins), int(kernel), float(velfac), _ptr(plan.pos), _ptr(vel),
+                                      _ptr(temp), _ptr(plan.h), arr, len(quads), _ptr(gT), _ptr(gv), _stream()))
+    return gd, gT, gv
 
 
 def colden(plan: Plan, nbins: int, kernel: int, velfac: float, dens: Sequence[torch.Tensor],

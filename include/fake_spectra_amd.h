@@ -74,6 +74,29 @@ int fsamd_tau(fsamd_plan *plan, int nbins, int kernel, double velfac, double tau
               const float *pos, const float *vel, const float *temp, const float *h,
               const fsamd_line *lines, int nlines, int mode, void *stream);
 
+/* Gradients of a scalar loss L through fsamd_tau (no counterpart in the
+ * reference, which has no gradients).  Before the forward call, enable
+ * fsamd_plan_keep_windows(plan, 1): the forward then records how many pixels
+ * each (line, pair, direction) deposited, and the adjoint visits exactly
+ * those pixels, holding the discrete choices (window ends, pixel index,
+ * sub-sample count) fixed.  For each line, grad_tau is dL/dtau (device
+ * [nlos, nbins] float64) and grad_dens receives dL/d dens (device [npart]
+ * float64, accumulated with +=); grad_temp [npart] and grad_vel [npart, 3]
+ * (float64, +=) receive dL/dT and dL/d vel summed over the lines.  The
+ * lines, nbins and kernel must match the recorded forward call. */
+typedef struct fsamd_line_grad {
+    double lambda_cm, gamma, fosc, amumass;
+    const float *dens;        /* device [npart] */
+    const double *grad_tau;   /* device [nlos, nbins] */
+    double *grad_dens;        /* device [npart], += */
+} fsamd_line_grad;
+
+int fsamd_plan_keep_windows(fsamd_plan *plan, int enable);
+int fsamd_tau_adjoint(fsamd_plan *plan, int nbins, int kernel, double velfac, const float *pos,
+                      const float *vel, const float *temp, const float *h,
+                      const fsamd_line_grad *lines, int nlines, double *grad_temp,
+                      double *grad_vel, void *stream);
+
 /* Column density (ParticleInterp::compute_colden, part_int.cpp:53-84, with
  * LineAbsorption::add_colden_particle, absorption.cpp:167-210).  Only dens
  * and out of each line are used. */
