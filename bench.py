@@ -141,6 +141,12 @@ def main():
     K = len(w.lines)
     torch.cuda.synchronize()
 
+    # pick the lane-group width on this workload (kept by the plan for later calls)
+    tplan = SP.Plan(w.box, pos, hh, ax, cofm)
+    best_g, tune_ms = SP.autotune(tplan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail)
+    tplan.close()
+    N.set_group_size(best_g)
+
     state = {}
 
     def step():
@@ -217,6 +223,22 @@ def main():
         "deposit_ms": dep_ms,
         "binning_ms": ms_per_step - dep_ms,
     }
+    # the same deposit captured once in a HIP graph and replayed
+    gplan = SP.Plan(w.box, pos, hh, ax, cofm)
+    graph = SP.TauGraph(gplan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    graph.replay()
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(args.steps):
+        graph.replay()
+    ev1.record()
+    torch.cuda.synchronize()
+    out["graph"] = {"deposit_ms": ev0.elapsed_time(ev1) / args.steps, "overflow": graph.overflowed(),
+                    "note": "batched deposit + fix-up pass replayed from one captured hipGraph (binning excluded)"}
+    del graph
+    gplan.close()
+    out["autotune"] = {"group_width": best_g, "sample_ms": {str(k): v for k, v in tune_ms.items()}}
     if not args.no_plain:
         plan = SP.Plan(w.box, pos, hh, ax, cofm)
         torch.cuda.synchronize()

@@ -24,7 +24,8 @@ MODE_PLAIN = 1
 EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan_destroy",
            "fsamd_plan_npairs", "fsamd_plan_pairs", "fsamd_plan_near_lines", "fsamd_tau",
            "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats", "fsamd_last_ms",
-           "fsamd_voigt_peak", "fsamd_set_group_size", "fsamd_plan_keep_windows", "fsamd_tau_adjoint")
+           "fsamd_voigt_peak", "fsamd_set_group_size", "fsamd_plan_keep_windows", "fsamd_tau_adjoint",
+           "fsamd_tau_autotune", "fsamd_plan_overflow")
 
 
 class FsamdLine(C.Structure):
@@ -79,6 +80,11 @@ def _declare(L):
     L.fsamd_plan_keep_windows.restype = i
     L.fsamd_tau_adjoint.argtypes = [vp, i, i, d, vp, vp, vp, vp, C.POINTER(FsamdLineGrad), i, vp, vp, vp]
     L.fsamd_tau_adjoint.restype = i
+    L.fsamd_tau_autotune.argtypes = [vp, i, i, d, d, vp, vp, vp, vp, C.POINTER(FsamdLine), i, vp, C.POINTER(i),
+                                     C.POINTER(C.c_float)]
+    L.fsamd_tau_autotune.restype = i
+    L.fsamd_plan_overflow.argtypes = [vp, C.POINTER(i), vp]
+    L.fsamd_plan_overflow.restype = i
 
 
 def lib():

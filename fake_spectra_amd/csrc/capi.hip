@@ -52,6 +52,18 @@ static void line_consts(const fsamd_line &l, LineDesc &d)
     d.out = l.out;
 }
 
+static bool find_item_set(const fsamd_plan *P, int K, long long chunk)
+{
+    for (auto &it : P->item_sets)
+        if (it->K == K && it->chunk == chunk) return true;
+    return false;
+}
+
+static long long tune_key(int nbins, int nlines, int kernel)
+{
+    return ((long long)nbins << 20) ^ ((long long)nlines << 4) ^ kernel;
+}
+
 // Work items for K lines in the plan's heaviest-first order; a sightline
 // with more than `chunk` pairs is split into non-direct items.  Cached.
 static int item_set(fsamd_plan *P, int K, long long chunk, fsamd_plan::ItemSet **out)
@@ -66,9 +78,13 @@ static int item_set(fsamd_plan *P, int K, long long chunk, fsamd_plan::ItemSet *
     set->chunk = chunk;
     set->split = false;
     std::vector<WorkItem> &items = set->host;
+    const long long stride = chunk < 0 ? -chunk : 1;
+    if (chunk < 0) chunk = 1ll << 62;
+    long long nsel = 0;
     for (int s : P->order_host) {
         const long long a = P->off_host[s], b = P->off_host[s + 1];
         const long long n = b - a;
+        if (nsel++ % stride) continue;
         for (int k = 0; k < K; ++k) {
             if (n <= chunk) {
                 items.push_back(WorkItem{s, k, a, b, 1, 0});
@@ -119,7 +135,8 @@ static int upload_lines(fsamd_plan *P, const std::vector<LineDesc> &ld, hipStrea
 
 static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double velfac, double tautail,
                        const float *pos, const float *vel, const float *temp, const float *h,
-                       const fsamd_line *lines, int nlines, int mode, hipStream_t s)
+                       const fsamd_line *lines, int nlines, int mode, hipStream_t s, long long sample_stride = 0,
+                       int force_g = 0)
 {
     if (!P) return fail(FSAMD_ERR_ARG, "plan is NULL");
     if (nbins < 1) return fail(FSAMD_ERR_ARG, "nbins must be positive");
@@ -148,11 +165,19 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         FSAMD_HIP(hipEventRecord(P->ev[0], s));
     }
     const bool plain = (mode == FSAMD_MODE_PLAIN);
-    const int gsize = g_group_size;
+    const int gsize = force_g ? force_g
+                      : (P->tuned_g > 0 && P->tuned_key == tune_key(nbins, nlines, kernel)) ? P->tuned_g : g_group_size;
+    const long long chunk_key = plain ? kPlainChunk : (sample_stride > 1 ? -sample_stride : (1ll << 62));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    FSAMD_HIP(hipStreamIsCapturing(s, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    if (capturing && (g_stats_on || plain))
+        return fail(FSAMD_ERR_ARG, "graph capture supports the batched mode without statistics only");
     const bool rec = tau && P->keep_windows;
     if (rec) {
         const size_t wbytes = sizeof(int) * 2 * (size_t)std::max<long long>(P->npairs, 1) * nlines;
         if (P->win.bytes < wbytes) {
+            if (capturing) return fail(FSAMD_ERR_ARG, "run one uncaptured fsamd_tau call before capturing");
             FSAMD_HIP(hipStreamSynchronize(s));
             if ((rc = P->win.alloc(wbytes))) return rc;
         }
@@ -164,6 +189,8 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
     if (tau) {
         // room for deferred FP64 walks: a few percent of pair-directions
         const long long want = std::max<long long>(65536, 2 * P->npairs * std::min(nlines, kMaxLines) / 32);
+        if ((P->fix_cap < want || !P->fix_count.ptr) && capturing)
+            return fail(FSAMD_ERR_ARG, "run one uncaptured fsamd_tau call before capturing");
         if (P->fix_cap < want && (rc = grow_fixups(P, want, s))) return rc;
         if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(int) * 2))) return rc;
     }
@@ -193,7 +220,9 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         A.npairs = P->npairs;
         A.win = rec ? P->win.as<int>() + (size_t)k0 * 2 * P->npairs : nullptr;
         fsamd_plan::ItemSet *set = nullptr;
-        if ((rc = item_set(P, K, plain ? kPlainChunk : (1ll << 62), &set))) return rc;
+        if (capturing && !find_item_set(P, K, chunk_key))
+            return fail(FSAMD_ERR_ARG, "run one uncaptured fsamd_tau call before capturing");
+        if ((rc = item_set(P, K, chunk_key, &set))) return rc;
         const std::vector<WorkItem> &items = set->host;
         for (int attempt = 0;; ++attempt) {
             A.fix = P->fix.as<Fixup>();
@@ -204,7 +233,8 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
                     if ((rc = launch_fill_f64(ld[k0 + k].out, 0.0, (long long)P->nlos * nbins, s))) return rc;
             }
             if (items.empty()) break;
-            if (tau && (rc = launch_fill_i32(A.fix_count, 0, 2, s))) return rc;
+            // captured graphs keep the overflow flag sticky across replays
+            if (tau && (rc = launch_fill_i32(A.fix_count, 0, capturing ? 1 : 2, s))) return rc;
             if (!plain) {
                 A.nitems = (int)items.size();
                 if ((rc = launch_fill_i32(A.queue, 0, 1, s))) return rc;
@@ -225,6 +255,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
             }
             if (!tau) break;
             if ((rc = launch_tau_fixup(A, set->buf.as<WorkItem>(), dev.ncu * 2, s))) return rc;
+            if (capturing) break;   // overflow is checked after replay (fsamd_plan_overflow)
             int fc[2] = {0, 0};
             FSAMD_HIP(hipMemcpyAsync(fc, A.fix_count, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
             FSAMD_HIP(hipStreamSynchronize(s));
@@ -338,6 +369,62 @@ int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double bo
     (void)hipStreamSynchronize((hipStream_t)stream);
     fsamd_plan_destroy(plan);
     return rc;
+}
+
+int fsamd_plan_overflow(fsamd_plan *plan, int *flag, void *stream)
+{
+    if (!plan || !flag) return fail(FSAMD_ERR_ARG, "NULL argument");
+    *flag = 0;
+    if (!plan->fix_count.ptr) return FSAMD_OK;
+    int fc[2] = {0, 0};
+    FSAMD_HIP(hipMemcpyAsync(fc, plan->fix_count.ptr, sizeof(int) * 2, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    FSAMD_HIP(hipStreamSynchronize((hipStream_t)stream));
+    *flag = fc[1];
+    return FSAMD_OK;
+}
+
+int fsamd_tau_autotune(fsamd_plan *P, int nbins, int kernel, double velfac, double tautail, const float *pos,
+                       const float *vel, const float *temp, const float *h, const fsamd_line *lines, int nlines,
+                       void *stream, int *best_g, float *ms3)
+{
+    if (!P || !best_g) return fail(FSAMD_ERR_ARG, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    // every 4th sightline of the heaviest-first order: the same mix of heavy
+    // and light work items, a quarter of the work
+    const long long stride = P->nlos >= 4096 ? 4 : 1;
+    const int widths[3] = {16, 32, 64};
+    hipEvent_t e0, e1;
+    FSAMD_HIP(hipEventCreate(&e0));
+    FSAMD_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bg = 16;
+    int rc = FSAMD_OK;
+    for (int rep = 0; rep < 2 && rc == FSAMD_OK; ++rep) {   // first pass warms caches and allocations
+        for (int i = 0; i < 3; ++i) {
+            FSAMD_HIP(hipEventRecord(e0, s));
+            rc = run_deposit(true, P, nbins, kernel, velfac, tautail, pos, vel, temp, h, lines, nlines,
+                             FSAMD_MODE_BATCHED, s, stride, widths[i]);
+            if (rc) break;
+            FSAMD_HIP(hipEventRecord(e1, s));
+            FSAMD_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            FSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep == 1) {
+                if (ms3) ms3[i] = ms;
+                if (ms < best) {
+                    best = ms;
+                    bg = widths[i];
+                }
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    P->tuned_g = bg;
+    P->tuned_key = tune_key(nbins, nlines, kernel);
+    *best_g = bg;
+    return FSAMD_OK;
 }
 
 int fsamd_plan_keep_windows(fsamd_plan *plan, int enable)

@@ -973,13 +973,23 @@ __global__ __launch_bounds__(kThreads) void voigt_peak_kernel(long long per_thre
 
 size_t deposit_lds_bytes(int nbins) { return sizeof(double) * (size_t)nbins; }
 
+// Raise a kernel's dynamic-LDS limit once, to the largest size requested so far.
+static int lds_limit(const void *fn, size_t shm, size_t &set_to)
+{
+    if (shm > set_to) {
+        FSAMD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        set_to = shm;
+    }
+    return FSAMD_OK;
+}
+
 template <int kG, bool kLds>
 static int launch_tau(const DepositArgs &A, const WorkItem *items, int item_base, int nblocks, int use_queue,
                       size_t shm, hipStream_t s)
 {
-    if (kLds)
-        FSAMD_HIP(hipFuncSetAttribute((const void *)tau_kernel<kG, kLds>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)shm));
+    static size_t set_to = 0;
+    int rc;
+    if (kLds && (rc = lds_limit((const void *)tau_kernel<kG, kLds>, shm, set_to))) return rc;
     tau_kernel<kG, kLds><<<nblocks, kThreads, shm, s>>>(A, items, item_base, use_queue);
     return FSAMD_OK;
 }

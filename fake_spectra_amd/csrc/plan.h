@@ -74,6 +74,9 @@ struct fsamd_plan {
     bool keep_windows = false;
     fsamd::DevBuf win;
     int win_lines = 0, win_nbins = 0, win_kernel = -1;
+    // lane-group width chosen by fsamd_tau_autotune for (nbins, nlines, kernel)
+    int tuned_g = 0;
+    long long tuned_key = -1;
     hipEvent_t ev[2] = {nullptr, nullptr};
     ~fsamd_plan()
     {

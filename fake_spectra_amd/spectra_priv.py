@@ -194,6 +194,55 @@ def tau_adjoint(plan: Plan, nbins: int, kernel: int, velfac: float, vel: torch.T
     return gd, gT, gv
 
 
+def autotune(plan: Plan, nbins: int, kernel: int, velfac: float, vel: torch.Tensor, temp: torch.Tensor,
+             lines: Sequence, dens: Sequence[torch.Tensor], tautail: float = 1e-7):
+    """Pick the tau kernel's lane-group width for this plan and call shape by
+    timing 16, 32 and 64 on a sample of the work; returns (width, {width: ms})."""
+    vel = _require(vel, torch.float32, "vel")
+    temp = _require(temp, torch.float32, "temp")
+    quads = [_as_quad(x) for x in lines]
+    dens = [_require(d, torch.float32, "dens") for d in dens]
+    scratch = [torch.empty((plan.nlos, nbins), dtype=torch.float64, device=plan.pos.device) for _ in quads]
+    arr = _line_structs(quads, dens, scratch)
+    best = C.c_int()
+    ms = (C.c_float * 3)()
+    N.check(N.lib().fsamd_tau_autotune(plan.handle, int(nbins), int(kernel), float(velfac), float(tautail),
+                                       _ptr(plan.pos), _ptr(vel), _ptr(temp), _ptr(plan.h), arr, len(quads), _stream(),
+                                       C.byref(best), ms))
+    return best.value, {16: ms[0], 32: ms[1], 64: ms[2]}
+
+
+class TauGraph:
+    """One batched tau call (all sightlines x lines in one persistent launch
+    plus its fix-up pass) captured in a HIP graph and replayed.  The outputs
+    are fixed tensors (self.out); inputs are read from the tensors given here
+    at every replay."""
+
+    def __init__(self, plan: Plan, nbins: int, kernel: int, velfac: float, vel: torch.Tensor,
+                 temp: torch.Tensor, lines: Sequence, dens: Sequence[torch.Tensor], tautail: float = 1e-7):
+        self.plan = plan
+        self.args = (nbins, kernel, velfac, _require(vel, torch.float32, "vel"),
+                     _require(temp, torch.float32, "temp"), list(lines),
+                     [_require(d, torch.float32, "dens") for d in dens], tautail)
+        self.out = [torch.empty((plan.nlos, nbins), dtype=torch.float64, device=plan.pos.device) for _ in lines]
+        nb, kern, vf, v, t, ln, d, tt = self.args
+        tau(plan, nb, kern, vf, v, t, ln, d, tt, out=self.out)      # allocations and tables, uncaptured
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            tau(plan, nb, kern, vf, v, t, ln, d, tt, out=self.out)
+
+    def replay(self):
+        self.graph.replay()
+        return self.out
+
+    def overflowed(self) -> bool:
+        """True if a replay overflowed the fix-up list (results then invalid)."""
+        flag = C.c_int()
+        N.check(N.lib().fsamd_plan_overflow(self.plan.handle, C.byref(flag), _stream()))
+        return bool(flag.value)
+
+
 def colden(plan: Plan, nbins: int, kernel: int, velfac: float, dens: Sequence[torch.Tensor],
            out: Optional[Sequence[torch.Tensor]] = None):
     """Column density float64 [nlos, nbins] for each density array."""

@@ -125,6 +125,24 @@ int fsamd_set_stats(int enable);
 int fsamd_last_stats(long long *out6);
 double fsamd_last_ms(void);
 
+/* Autotuner (no counterpart in the reference): times the batched tau launch
+ * for lane-group widths 16, 32 and 64 on every 4th sightline of the plan's
+ * heaviest-first order (all of them below 4096 sightlines) and keeps the
+ * fastest for later fsamd_tau calls with the same nbins, line count and
+ * kernel.  Writes partial results into the lines' outputs.  ms3 (optional,
+ * host) receives the three times. */
+int fsamd_tau_autotune(fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail,
+                       const float *pos, const float *vel, const float *temp, const float *h,
+                       const fsamd_line *lines, int nlines, void *stream, int *best_g, float *ms3);
+
+/* fsamd_tau inside a hipGraph capture (the stream reports capturing):
+ * allowed in the batched mode without statistics after one uncaptured call
+ * with the same plan, lines and sizes; the captured sequence allocates,
+ * copies and synchronizes nothing.  A fix-up list overflow cannot be handled
+ * inside a replay: it sets a sticky flag that fsamd_plan_overflow reads
+ * (synchronizing the stream); if set, rerun the call uncaptured. */
+int fsamd_plan_overflow(fsamd_plan *plan, int *flag, void *stream);
+
 /* Tuning knob with no counterpart in the reference: pixels walked per lane
  * group of the tau kernel (16, 32 or 64; a 64-lane wavefront walks 64/g
  * (pair, direction) streams at once).  Process-wide; default 16. */

@@ -120,6 +120,40 @@ def test_group_sizes_match_oracle(oracle, g):
         assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)), "G=%d %s" % (g, ln.name))
 
 
+def test_graph_replays_match_eager():
+    """A captured batched launch, replayed several times with eager work and
+    synchronizations in between, reproduces the eager result every time."""
+    w = synth.config(1, scale=1 / 64, nlos=12)
+    plan = plan_for(w)
+    vel, temp = dev(w.vel), dev(w.temp)
+    dens = [dev(w.dens_for(ln)) for ln in w.lines]
+    eager = [o.clone() for o in SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)]
+    g = SP.TauGraph(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
+    for rep in range(4):
+        for o in g.out:
+            o.fill_(-1.0)
+        torch.cuda.synchronize()
+        outs = g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(outs, eager):
+            np.testing.assert_allclose(a.cpu().numpy(), b.cpu().numpy(), rtol=1e-12, atol=1e-300)
+        # eager work between replays
+        SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
+    assert not g.overflowed()
+
+
+def test_autotune_picks_a_width(oracle):
+    w = synth.config(1, scale=1 / 64, nlos=8)
+    plan = plan_for(w)
+    vel, temp = dev(w.vel), dev(w.temp)
+    dens = [dev(w.dens_for(ln)) for ln in w.lines]
+    best, ms = SP.autotune(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
+    assert best in (16, 32, 64) and all(v > 0 for v in ms.values())
+    outs = SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
+    for ln, o in zip(w.lines, outs):
+        assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)), "tuned " + ln.name)
+
+
 def test_plain_mode_equals_batched():
     w = synth.config(2, scale=1 / 64, nlos=4)
     plan = plan_for(w)
