@@ -35,7 +35,22 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "copies": 3}))
 
 
 def cpu_baseline(w, budget_s):
@@ -124,6 +139,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each sightline is independent; "
               "see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import numpy as np
     import torch
     from fake_spectra_amd import synth, spectra_priv as SP
@@ -158,7 +176,9 @@ def main():
         state["stats"] = st
         plan.close()
 
-    SP.set_stats(True)
+    # timed steps carry only hipEvents around the deposit; the counters come
+    # from one more, untimed step
+    SP.set_stats(True, timing_only=True)
     for _ in range(args.warmup):
         step()
     state["ms"] = []
@@ -168,12 +188,14 @@ def main():
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    dep_ms = float(np.mean(state["ms"]))
+    SP.set_stats(True)
+    step()
     SP.set_stats(False)
 
     ms_per_step = 1e3 * elapsed / args.steps
     units = K * w.nlos * w.nbins
     value = units / (elapsed / args.steps)
-    dep_ms = float(np.mean(state["ms"]))
     M = state["npairs"]
     st = state["stats"]
     # algorithmic HBM bytes of one deposit launch: per (pair, line) the pair
@@ -218,7 +240,8 @@ def main():
                  "isolated_peak_samples_per_s": peak, "frac": samples_per_s / peak,
                  "voigt_samples_per_launch": st["voigt_samples"],
                  "pixels_deposited": st["pixels"], "pixels_evaluated": st["evaluated_pixels"],
-                 "fp64_tailcut_pixels": st["fp64_pixels"], "fp64_pairs": st["fp64_pairs"]},
+                 "fp64_tailcut_pixels": st["fp64_pixels"], "fp64_pairs": st["fp64_pairs"],
+                 "kernel_sums": st["kernel_sums"], "far_wing_pixels": st["far_pixels"]},
         "pairs": M,
         "deposit_ms": dep_ms,
         "binning_ms": ms_per_step - dep_ms,

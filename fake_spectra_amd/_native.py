@@ -68,7 +68,7 @@ def _declare(L):
     L.fsamd_particle_interpolate.restype = i
     L.fsamd_set_stats.argtypes = [i]
     L.fsamd_set_stats.restype = i
-    L.fsamd_last_stats.argtypes = [C.POINTER(ll)]
+    L.fsamd_last_stats.argtypes = [C.POINTER(ll), i]
     L.fsamd_last_stats.restype = i
     L.fsamd_last_ms.argtypes = []
     L.fsamd_last_ms.restype = d

@@ -11,8 +11,9 @@
 namespace fsamd {
 
 static thread_local std::string g_error;
-static thread_local long long g_stats[kNStats] = {0, 0, 0, 0, 0, 0};
-static thread_local int g_stats_on = 0;
+static thread_local long long g_stats[kNStats] = {};
+static thread_local int g_stats_on = 0;    // device counters (and timing)
+static thread_local int g_timing_on = 0;   // hipEvents around the launches
 static thread_local double g_last_ms = 0;
 static int g_group_size = 16;   // pixels per lane group of the tau kernel (16, 32 or 64)
 
@@ -105,31 +106,60 @@ static int item_set(fsamd_plan *P, int K, long long chunk, fsamd_plan::ItemSet *
     return FSAMD_OK;
 }
 
+// (Re)allocate a launch-state buffer.  A buffer that a captured graph may
+// still address is retired rather than freed.
+static int regrow(fsamd_plan *P, DevBuf &b, size_t bytes)
+{
+    if (P->captured && b.ptr) P->retired.push_back(b.disown());
+    return b.alloc(bytes);
+}
+
 static int grow_fixups(fsamd_plan *P, long long cap, hipStream_t s)
 {
     cap = std::min<long long>(cap, INT32_MAX / 2);
     FSAMD_HIP(hipStreamSynchronize(s));
-    int rc = P->fix.alloc(sizeof(Fixup) * (size_t)cap);
+    int rc = regrow(P, P->fix, sizeof(Fixup) * (size_t)cap);
     if (rc) return rc;
     P->fix_cap = (int)cap;
     return FSAMD_OK;
 }
 
-// Upload the line table only when it changed since the last call.
-static int upload_lines(fsamd_plan *P, const std::vector<LineDesc> &ld, hipStream_t s)
+// The device LineDesc table for this call.  Eager calls rewrite the shared
+// table when the lines (or their output pointers) change; a captured call
+// needs a table with exactly its contents made by an earlier eager call, and
+// pins it so that later calls never rewrite what the graph reads.
+static int line_table(fsamd_plan *P, const std::vector<LineDesc> &ld, bool capturing, hipStream_t s,
+                      const LineDesc **out)
 {
     const size_t bytes = sizeof(LineDesc) * ld.size();
-    if (P->lines_dev.bytes < bytes) {
-        FSAMD_HIP(hipStreamSynchronize(s));
-        int rc = P->lines_dev.alloc(bytes);
-        if (rc) return rc;
-        P->lines_host.clear();
-    }
-    if (P->lines_host.size() == ld.size() && std::memcmp(P->lines_host.data(), ld.data(), bytes) == 0)
+    auto same = [&](const fsamd_plan::LineTable &t) {
+        return t.host.size() == ld.size() && std::memcmp(t.host.data(), ld.data(), bytes) == 0;
+    };
+    for (auto &t : P->pinned_lines)
+        if (same(*t)) {
+            *out = t->buf.as<LineDesc>();
+            return FSAMD_OK;
+        }
+    fsamd_plan::LineTable &e = P->eager_lines;
+    if (capturing) {
+        if (!same(e)) return fail(FSAMD_ERR_ARG, "run one uncaptured call with the same lines and outputs before capturing");
+        auto t = std::make_unique<fsamd_plan::LineTable>();
+        t->buf.take(e.buf);
+        t->host.swap(e.host);
+        *out = t->buf.as<LineDesc>();
+        P->pinned_lines.push_back(std::move(t));
         return FSAMD_OK;
-    FSAMD_HIP(hipStreamSynchronize(s));   // the previous table may still be in use
-    FSAMD_HIP(hipMemcpy(P->lines_dev.ptr, ld.data(), bytes, hipMemcpyHostToDevice));
-    P->lines_host = ld;
+    }
+    if (!same(e)) {
+        FSAMD_HIP(hipStreamSynchronize(s));   // the previous table may still be in use
+        if (e.buf.bytes < bytes) {
+            int rc = e.buf.alloc(bytes);
+            if (rc) return rc;
+        }
+        FSAMD_HIP(hipMemcpy(e.buf.ptr, ld.data(), bytes, hipMemcpyHostToDevice));
+        e.host = ld;
+    }
+    *out = e.buf.as<LineDesc>();
     return FSAMD_OK;
 }
 
@@ -153,33 +183,39 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
     const bool lds = deposit_lds_bytes(nbins) + 2048 <= (size_t)dev.max_lds;
     const int ngroups = (nlines + kMaxLines - 1) / kMaxLines;
 
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    FSAMD_HIP(hipStreamIsCapturing(s, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    const bool plain = (mode == FSAMD_MODE_PLAIN);
+    if (capturing && (g_timing_on || plain))
+        return fail(FSAMD_ERR_ARG, "graph capture supports the batched mode without statistics only");
     std::vector<LineDesc> ld(nlines);
     for (int i = 0; i < nlines; ++i) line_consts(lines[i], ld[i]);
-    if ((rc = upload_lines(P, ld, s))) return rc;
-    if (P->queue.bytes < sizeof(int) * (size_t)ngroups && (rc = P->queue.alloc(sizeof(int) * ngroups))) return rc;
+    const LineDesc *table = nullptr;
+    if ((rc = line_table(P, ld, capturing, s, &table))) return rc;
+    if (P->queue.bytes < sizeof(int) * (size_t)ngroups) {
+        if (capturing) return fail(FSAMD_ERR_ARG, "run one uncaptured call before capturing");
+        if ((rc = regrow(P, P->queue, sizeof(int) * ngroups))) return rc;
+    }
     if (g_stats_on) {
         if (!P->stats.ptr && (rc = P->stats.alloc(sizeof(long long) * kNStats))) return rc;
         if ((rc = launch_fill_f64((double *)P->stats.ptr, 0.0, kNStats, s))) return rc;   // all-zero bits
+    }
+    if (g_timing_on) {
         for (auto &e : P->ev)
             if (!e) FSAMD_HIP(hipEventCreate(&e));
         FSAMD_HIP(hipEventRecord(P->ev[0], s));
     }
-    const bool plain = (mode == FSAMD_MODE_PLAIN);
     const int gsize = force_g ? force_g
                       : (P->tuned_g > 0 && P->tuned_key == tune_key(nbins, nlines, kernel)) ? P->tuned_g : g_group_size;
     const long long chunk_key = plain ? kPlainChunk : (sample_stride > 1 ? -sample_stride : (1ll << 62));
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    FSAMD_HIP(hipStreamIsCapturing(s, &cap));
-    const bool capturing = cap != hipStreamCaptureStatusNone;
-    if (capturing && (g_stats_on || plain))
-        return fail(FSAMD_ERR_ARG, "graph capture supports the batched mode without statistics only");
     const bool rec = tau && P->keep_windows;
     if (rec) {
         const size_t wbytes = sizeof(int) * 2 * (size_t)std::max<long long>(P->npairs, 1) * nlines;
         if (P->win.bytes < wbytes) {
             if (capturing) return fail(FSAMD_ERR_ARG, "run one uncaptured fsamd_tau call before capturing");
             FSAMD_HIP(hipStreamSynchronize(s));
-            if ((rc = P->win.alloc(wbytes))) return rc;
+            if ((rc = regrow(P, P->win, wbytes))) return rc;
         }
         if ((rc = launch_fill_i32(P->win.as<int>(), 0, (long long)(wbytes / sizeof(int)), s))) return rc;
         P->win_lines = nlines;
@@ -194,6 +230,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         if (P->fix_cap < want && (rc = grow_fixups(P, want, s))) return rc;
         if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(int) * 2))) return rc;
     }
+    if (capturing) P->captured = true;
     for (int g = 0; g < ngroups; ++g) {
         const int k0 = g * kMaxLines;
         const int K = std::min(kMaxLines, nlines - k0);
@@ -214,7 +251,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         A.line_off = P->line_off.as<long long>();
         A.pair_part = P->pair_part.as<int>();
         A.nlines = K;
-        A.lines = P->lines_dev.as<LineDesc>() + k0;
+        A.lines = table + k0;
         A.queue = P->queue.as<int>() + g;
         A.stats = g_stats_on ? P->stats.as<long long>() : nullptr;
         A.npairs = P->npairs;
@@ -266,9 +303,10 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
             if (g_stats_on && (rc = launch_fill_f64((double *)P->stats.ptr, 0.0, kNStats, s))) return rc;
         }
     }
-    if (g_stats_on) {
+    if (g_timing_on) {
         FSAMD_HIP(hipEventRecord(P->ev[1], s));
-        FSAMD_HIP(hipMemcpyAsync(g_stats, P->stats.ptr, sizeof(long long) * kNStats, hipMemcpyDeviceToHost, s));
+        if (g_stats_on)
+            FSAMD_HIP(hipMemcpyAsync(g_stats, P->stats.ptr, sizeof(long long) * kNStats, hipMemcpyDeviceToHost, s));
         FSAMD_HIP(hipStreamSynchronize(s));
         float ms = 0;
         FSAMD_HIP(hipEventElapsedTime(&ms, P->ev[0], P->ev[1]));
@@ -389,9 +427,11 @@ int fsamd_tau_autotune(fsamd_plan *P, int nbins, int kernel, double velfac, doub
 {
     if (!P || !best_g) return fail(FSAMD_ERR_ARG, "NULL argument");
     hipStream_t s = (hipStream_t)stream;
-    // every 4th sightline of the heaviest-first order: the same mix of heavy
-    // and light work items, a quarter of the work
-    const long long stride = P->nlos >= 4096 ? 4 : 1;
+    // Large plans are timed on every stride-th sightline of the heaviest-first
+    // order (the same mix of heavy and light work items), at least 16384
+    // sightlines so that each block still drains several items: with fewer,
+    // the last items dominate and the widths time within noise of each other.
+    const long long stride = std::max<long long>(1, P->nlos / 16384);
     const int widths[3] = {16, 32, 64};
     hipEvent_t e0, e1;
     FSAMD_HIP(hipEventCreate(&e0));
@@ -399,8 +439,8 @@ int fsamd_tau_autotune(fsamd_plan *P, int nbins, int kernel, double velfac, doub
     float best = 1e30f;
     int bg = 16;
     int rc = FSAMD_OK;
-    for (int rep = 0; rep < 2 && rc == FSAMD_OK; ++rep) {   // first pass warms caches and allocations
-        for (int i = 0; i < 3; ++i) {
+    for (int rep = 0; rep < 2 && rc == FSAMD_OK; ++rep) {   // one untimed run makes the allocations
+        for (int i = 0; i < (rep ? 3 : 1); ++i) {
             FSAMD_HIP(hipEventRecord(e0, s));
             rc = run_deposit(true, P, nbins, kernel, velfac, tautail, pos, vel, temp, h, lines, nlines,
                              FSAMD_MODE_BATCHED, s, stride, widths[i]);
@@ -462,9 +502,13 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         line_consts(l, ld[i]);
         ld[i].grad_dens = lines[i].grad_dens;
     }
-    if ((rc = upload_lines(P, ld, s))) return rc;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    FSAMD_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) return fail(FSAMD_ERR_ARG, "fsamd_tau_adjoint cannot be graph-captured");
+    const LineDesc *table = nullptr;
+    if ((rc = line_table(P, ld, false, s, &table))) return rc;
     const int ngroups = (nlines + kMaxLines - 1) / kMaxLines;
-    if (P->queue.bytes < sizeof(int) * (size_t)ngroups && (rc = P->queue.alloc(sizeof(int) * ngroups))) return rc;
+    if (P->queue.bytes < sizeof(int) * (size_t)ngroups && (rc = regrow(P, P->queue, sizeof(int) * ngroups))) return rc;
     const int gsize = g_group_size;
     for (int g = 0; g < ngroups; ++g) {
         const int k0 = g * kMaxLines;
@@ -485,7 +529,7 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         A.line_off = P->line_off.as<long long>();
         A.pair_part = P->pair_part.as<int>();
         A.nlines = K;
-        A.lines = P->lines_dev.as<LineDesc>() + k0;
+        A.lines = table + k0;
         A.queue = P->queue.as<int>() + g;
         A.npairs = P->npairs;
         A.win = P->win.as<int>() + (size_t)k0 * 2 * P->npairs;
@@ -502,16 +546,18 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
     return FSAMD_OK;
 }
 
-int fsamd_set_stats(int enable)
+int fsamd_set_stats(int flags)
 {
-    g_stats_on = enable ? 1 : 0;
+    if (flags < 0 || flags > 3) return fail(FSAMD_ERR_ARG, "flags must be 0..3");
+    g_stats_on = flags & 1;
+    g_timing_on = flags != 0;
     return FSAMD_OK;
 }
 
-int fsamd_last_stats(long long *out6)
+int fsamd_last_stats(long long *out, int n)
 {
-    if (!out6) return fail(FSAMD_ERR_ARG, "NULL argument");
-    for (int i = 0; i < kNStats; ++i) out6[i] = g_stats[i];
+    if (!out || n < 0) return fail(FSAMD_ERR_ARG, "NULL argument or negative count");
+    for (int i = 0; i < std::min(n, kNStats); ++i) out[i] = g_stats[i];
     return FSAMD_OK;
 }
 

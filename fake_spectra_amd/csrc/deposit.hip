@@ -33,7 +33,7 @@
 
 namespace fsamd {
 
-__constant__ float c_gtab[kGTableSize] = FSAMD_G_COEF_INIT;
+__constant__ __attribute__((aligned(16))) float c_gtab[kGTableSize] = FSAMD_G_COEF_INIT;
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
@@ -370,7 +370,7 @@ __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const
 }
 
 struct LaneStats {
-    long long pairs = 0, pixels = 0, px64 = 0, pairs64 = 0, samples = 0, evaluated = 0;
+    long long pairs = 0, pixels = 0, px64 = 0, pairs64 = 0, samples = 0, evaluated = 0, sums = 0, far = 0;
 };
 
 // Voigt samples the reference evaluates for one pixel (singleabs.h:110-125).
@@ -418,7 +418,7 @@ __device__ inline unsigned long long group_bits(unsigned long long m, int lane)
 // One step of kG consecutive pixels of a walk (lane glane of the group);
 // returns true when the walk is over (tail cut reached, end of the window, or
 // the decision deferred to the fix-up kernel).
-template <int kG, int kMode, bool kLds>
+template <int kG, int kMode, bool kLds, bool kStats>
 __device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double *row, const float *gtab, int z0,
                                   int dir, int count, int c, int lane, LaneStats &st, long long k, int item,
                                   bool active, int &walked)
@@ -434,11 +434,17 @@ __device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double
     if (valid) {
         vlow = z * bintov - P.vel;                                  // absorption.cpp:252
         t = taulast_pixel<kMode == kWalkFull64>(P, vlow, bintov, A.velfac, A.kernel, gtab);
-        st.evaluated++;
+        if (kStats) {
+            st.evaluated++;
+            if (kMode != kWalkFull64 && fabs((vlow + 0.5 * bintov) * P.inv_b) >= P.ufar)
+                st.far++;
+            else
+                st.sums += pixel_samples(P, vlow, bintov) / (kNGrid - 1);
+        }
         if (kMode != kWalkFull64 && fabs(t - tt) <= kGuard * tt) {
             if (kMode == kWalkGuarded) {
                 t = taulast_pixel<true>(P, vlow, bintov, A.velfac, A.kernel, gtab);
-                st.px64++;
+                if (kStats) st.px64++;
             } else {
                 guard = true;
             }
@@ -450,8 +456,10 @@ __device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double
     const int first = mall ? __builtin_ctzll(mall) : kG;
     if (valid && (glane < first || (glane == first && ((mb >> glane) & 1ull)))) {
         row_add<kLds>(&row[wrap_pixel(z, A.nbins)], t);
-        st.pixels++;
-        if (A.stats) st.samples += pixel_samples(P, vlow, bintov);
+        if (kStats) {
+            st.pixels++;
+            st.samples += pixel_samples(P, vlow, bintov);
+        }
     }
     if (kMode == kWalkMain && first < kG && ((mg >> first) & 1ull)) {
         // the FP32 value of the deciding pixel is too close to tautail
@@ -500,21 +508,22 @@ __device__ inline int walk64(const DepositArgs &A, PairTau &P, double *row, cons
             if (lane == 0) P.wing_ready = 1;
             wave_sync();
         }
-        if (walk_chunk<64, kMode, false>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item, true, walked))
+        if (walk_chunk<64, kMode, false, true>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item, true, walked))
             return walked;
     }
     return walked;
 }
 
 // 4 waves per SIMD: measured 8% faster at c1 than the 3 the register count
-// allows by default (a few dwords of spill in the pair setup).
-template <int kG, bool kLds>
+// allows by default (a few dwords of spill in the pair setup).  kStats adds
+// the per-lane event counts (A.stats set).
+template <int kG, bool kLds, bool kStats>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void tau_kernel(DepositArgs A, const WorkItem *items, int item_base,
                                                        int use_queue)
 {
     constexpr int kGroups = kThreads / kG;
     extern __shared__ double smem[];
-    __shared__ float gtab[kGTableSize];
+    __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kGroups];
     __shared__ int s_item;
     __shared__ unsigned long long s_next;
@@ -562,9 +571,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     dir = 2;
                 } else if (setup_pair<kG>(A, L, los, A.pair_part[kk], P, glane)) {
                     k = (long long)kk;
-                    st.pairs += (glane == 0);
+                    if (kStats) st.pairs += (glane == 0);
                     if (P.fp64) {
-                        st.pairs64 += (glane == 0);
+                        if (kStats) st.pairs64 += (glane == 0);
                         if (glane == 0) push_fixup(A, k, gitem, P.zmax, half, 0);
                     } else {
                         dir = 1;
@@ -581,7 +590,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 wave_sync();
             }
             int walked = 0;
-            const bool done = walk_chunk<kG, kWalkMain, kLds>(A, P, row, gtab, z0, dir, half, c, lane, st, k, gitem,
+            const bool done = walk_chunk<kG, kWalkMain, kLds, kStats>(A, P, row, gtab, z0, dir, half, c, lane, st, k, gitem,
                                                               walking, walked);
             if (walking) {
                 if (done && walked >= 0 && glane == 0) record_window(A, w.line, k, dir, walked);
@@ -608,12 +617,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             }
         }
     }
-    if (A.stats) {
+    if (kStats && A.stats) {
         atomicAdd((unsigned long long *)&A.stats[0], (unsigned long long)st.pairs);
         atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.pixels);
         atomicAdd((unsigned long long *)&A.stats[3], (unsigned long long)st.pairs64);
         atomicAdd((unsigned long long *)&A.stats[4], (unsigned long long)st.samples);
         atomicAdd((unsigned long long *)&A.stats[5], (unsigned long long)st.evaluated);
+        atomicAdd((unsigned long long *)&A.stats[6], (unsigned long long)st.sums);
+        atomicAdd((unsigned long long *)&A.stats[7], (unsigned long long)st.far);
     }
 }
 
@@ -621,7 +632,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 // fix-up entry, deposits straight into the output with FP64 atomics.
 __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, const WorkItem *items)
 {
-    __shared__ float gtab[kGTableSize];
+    __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kWaves];
     for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
     __syncthreads();
@@ -654,6 +665,8 @@ __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, cons
         atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st.px64);
         atomicAdd((unsigned long long *)&A.stats[4], (unsigned long long)st.samples);
         atomicAdd((unsigned long long *)&A.stats[5], (unsigned long long)st.evaluated);
+        atomicAdd((unsigned long long *)&A.stats[6], (unsigned long long)st.sums);
+        atomicAdd((unsigned long long *)&A.stats[7], (unsigned long long)st.far);
     }
 }
 
@@ -746,7 +759,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 {
     constexpr int kGroups = kThreads / kG;
     extern __shared__ double grow[];   // dL/dtau of the item's sightline and line
-    __shared__ float gtab[kGTableSize];
+    __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kGroups];
     __shared__ int s_item;
     __shared__ unsigned long long s_next;
@@ -950,7 +963,7 @@ __global__ __launch_bounds__(kThreads) void colden_kernel(DepositArgs A, const W
 __global__ __launch_bounds__(kThreads) void voigt_peak_kernel(long long per_thread, double du, double umax,
                                                               float a, float *sink)
 {
-    __shared__ float gtab[kGTableSize];
+    __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
     __syncthreads();
     const VoigtA va = make_voigt_a(a);
@@ -983,15 +996,23 @@ static int lds_limit(const void *fn, size_t shm, size_t &set_to)
     return FSAMD_OK;
 }
 
+template <int kG, bool kLds, bool kStats>
+static int launch_tau_k(const DepositArgs &A, const WorkItem *items, int item_base, int nblocks, int use_queue,
+                        size_t shm, hipStream_t s)
+{
+    static size_t set_to = 0;
+    int rc;
+    if (kLds && (rc = lds_limit((const void *)tau_kernel<kG, kLds, kStats>, shm, set_to))) return rc;
+    tau_kernel<kG, kLds, kStats><<<nblocks, kThreads, shm, s>>>(A, items, item_base, use_queue);
+    return FSAMD_OK;
+}
+
 template <int kG, bool kLds>
 static int launch_tau(const DepositArgs &A, const WorkItem *items, int item_base, int nblocks, int use_queue,
                       size_t shm, hipStream_t s)
 {
-    static size_t set_to = 0;
-    int rc;
-    if (kLds && (rc = lds_limit((const void *)tau_kernel<kG, kLds>, shm, set_to))) return rc;
-    tau_kernel<kG, kLds><<<nblocks, kThreads, shm, s>>>(A, items, item_base, use_queue);
-    return FSAMD_OK;
+    return A.stats ? launch_tau_k<kG, kLds, true>(A, items, item_base, nblocks, use_queue, shm, s)
+                   : launch_tau_k<kG, kLds, false>(A, items, item_base, nblocks, use_queue, shm, s);
 }
 
 int launch_deposit(bool tau, bool lds, int gsize, const DepositArgs &A, const WorkItem *items, int item_base,
@@ -1042,8 +1063,8 @@ template <int kG>
 static int occ_tau(bool lds, size_t shm)
 {
     int nb = 0;
-    hipError_t e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, true>, kThreads, shm)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, false>, kThreads, 0);
+    hipError_t e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, true, false>, kThreads, shm)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, false, false>, kThreads, 0);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return 1;

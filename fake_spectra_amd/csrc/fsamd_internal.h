@@ -77,7 +77,7 @@ struct Fixup {
     int count;
     int dir;
 };
-constexpr int kNStats = 6;
+constexpr int kNStats = FSAMD_NSTATS;
 
 struct DepositArgs {
     int nbins;

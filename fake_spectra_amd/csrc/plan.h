@@ -28,6 +28,14 @@ struct DevBuf {
     }
     template <typename T>
     T *as() const { return (T *)ptr; }
+    // Hand the allocation over to the caller (who frees it), leaving this empty.
+    void *disown()
+    {
+        void *p = ptr;
+        ptr = nullptr;
+        bytes = 0;
+        return p;
+    }
 };
 
 int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long long npart,
@@ -63,8 +71,15 @@ struct fsamd_plan {
     };
     std::vector<std::unique_ptr<ItemSet>> item_sets;
     fsamd::DevBuf queue;        // work-queue counters, one per launch group
-    fsamd::DevBuf lines_dev;    // LineDesc table
-    std::vector<fsamd::LineDesc> lines_host;
+    // LineDesc tables.  Eager calls share one table, rewritten when the lines
+    // change; a captured call pins the table it reads, which is then never
+    // rewritten or freed before the plan.
+    struct LineTable {
+        fsamd::DevBuf buf;
+        std::vector<fsamd::LineDesc> host;
+    };
+    LineTable eager_lines;
+    std::vector<std::unique_ptr<LineTable>> pinned_lines;
     fsamd::DevBuf stats;        // kNStats counters
     fsamd::DevBuf fix;          // fix-up list (FP64 walks)
     fsamd::DevBuf fix_count;    // [count, overflow] per line group
@@ -78,9 +93,15 @@ struct fsamd_plan {
     int tuned_g = 0;
     long long tuned_key = -1;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // Once a call on this plan has been captured into a graph, buffers that
+    // graph's kernels address are retired (kept until the plan is destroyed)
+    // instead of freed when a later call needs them larger.
+    bool captured = false;
+    std::vector<void *> retired;
     ~fsamd_plan()
     {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (void *p : retired) (void)hipFree(p);
     }
 };

@@ -5,20 +5,25 @@
 // deposit kernel evaluates it in FP32 instead, for damping parameters
 // 0 <= a <= kA32Max, from two expansions:
 //
-//  core, |u| < 7:  H = exp(a^2 - u^2) cos(2au) - (2/sqrt(pi)) [a G(u) - (a^3/6) D'''(u)]
-//     with G = 1 - 2u D(u) = D'(u) (D: Dawson's integral) taken from a
-//     piecewise polynomial table, and D''' = (4u^2 - 6) G + 2.  This is the
-//     Taylor series of w about the real axis, whose even terms sum to
-//     Re exp(-z^2) and whose odd terms are those of the entire function
-//     (2i/sqrt(pi)) D(z).  The a^5 term is below 1e-7 relative for a <= 0.04.
-//  wing, |u| >= 7: the asymptotic series w ~ (i/sqrt(pi)) sum_n (2n-1)!!/2^n z^-(2n+1)
-//     expanded in eps = a/u:  H = (a x / sqrt(pi)) (P(x) - a^2 x Q(x)), x = 1/u^2,
-//     P = sum (2n+1)!!/2^n x^n (9 terms), Q = sum C(2n+3,3) (2n-1)!!/2^n x^n (3 terms).
+//    H = exp(a^2 - u^2) cos(2au) - (2a/sqrt(pi)) G(u) - (a^3/sqrt(pi)) Z(u)
+//  with G = 1 - 2u D(u) = D'(u) (D: Dawson's integral).  This is the Taylor
+//  series of w about the real axis, whose even terms sum to Re exp(-z^2) and
+//  whose odd terms are those of the entire function (2i/sqrt(pi)) D(z); the
+//  a^5 term is below 1e-7 relative for a <= 0.04.
+//  |u| < 7:  G from a piecewise polynomial table, Z = -D'''/3 with
+//            D''' = (4u^2 - 6) G + 2;
+//  |u| >= 7: G = x R(x), x = 1/u^2, R a polynomial (the last table row) that
+//            follows the asymptotic series w ~ (i/sqrt(pi)) sum_n (2n-1)!!/2^n
+//            z^-(2n+1); Z = x^2 Q(x), Q = sum C(2n+3,3) (2n-1)!!/2^n x^n (3 terms),
+//            the a^3 term of the same series, free of the cancellation in D'''.
+//  Both forms are evaluated by every lane and selected, so the wavefront
+//  never diverges on |u|.
 //
-// The argument arrives as a double-float pair (uh, ul) because exp(-u^2)
-// amplifies the relative error of u by 2u^2; u^2 and the exponent are then
-// carried in double-float too, and exp uses the hardware 2^f on |f| <= 1/2.
-// Measured against scipy.special.wofz: <= 4e-7 relative for a <= 0.03.
+// u arrives in FP64, and u^2 and the exponent of exp(a^2 - u^2) stay in FP64
+// because exp amplifies the relative error of u by 2u^2; only 2^f on
+// |f| <= 1/2 runs in FP32 (the hardware exp2).  Measured against
+// scipy.special.wofz (the FP32 arithmetic emulated on the CPU): <= 3.6e-7
+// relative for a <= 0.04, 0 <= u <= 3000.
 //
 // An FP64 Faddeeva w(z) (voigt64) backs the FP32 path: pixels whose tail-cut
 // decision sits within a guard band of tautail, and pairs with a > kA32Max.
@@ -38,7 +43,8 @@ constexpr float kTwoOverSqrtPiF = 1.12837916709551257390f;
 constexpr float kInvSqrtPiF = 0.56418958354775628695f;
 constexpr double kSqrtPi = 1.77245385090551602729;
 constexpr double kTwoOverSqrtPi = 1.12837916709551257390;
-constexpr int kGTableSize = FSAMD_G_NPIECE * (FSAMD_G_DEG + 1);
+constexpr double kLog2E = 1.44269504088896340736;
+constexpr int kGTableSize = FSAMD_G_NROW * FSAMD_G_STRIDE;
 
 // (2n+1)!!/2^n, n = 0..8
 #define FSAMD_WING_P {1.0f, 1.5f, 3.75f, 13.125f, 59.0625f, 324.84375f, 2111.484375f, 15836.1328125f, 134607.12890625f}
@@ -53,6 +59,9 @@ struct VoigtA {
     float ea2;          // exp(a^2)
     float two_a_sqpi;   // 2a/sqrt(pi)
     float a_sqpi;       // a/sqrt(pi)
+    float a3_sqpi;      // a^3/sqrt(pi)
+    float two_a;        // 2a
+    double a2_log2e;    // a^2 log2(e)
 };
 
 __device__ inline VoigtA make_voigt_a(double a)
@@ -64,6 +73,9 @@ __device__ inline VoigtA make_voigt_a(double a)
     v.ea2 = (float)exp(a * a);
     v.two_a_sqpi = (float)(a * kTwoOverSqrtPi);
     v.a_sqpi = (float)(a / kSqrtPi);
+    v.a3_sqpi = (float)(a * a * a / kSqrtPi);
+    v.two_a = (float)(2.0 * a);
+    v.a2_log2e = a * a * kLog2E;
     return v;
 }
 
@@ -73,7 +85,7 @@ __device__ inline float g_of_u(float u, const float *tab)
     int idx = (int)(u * (1.0f / FSAMD_G_WIDTH));
     idx = idx < FSAMD_G_NPIECE - 1 ? idx : FSAMD_G_NPIECE - 1;
     const float t = __builtin_fmaf(u, 2.0f / FSAMD_G_WIDTH, -(2.0f * (float)idx + 1.0f));
-    const float *c = tab + idx * (FSAMD_G_DEG + 1);
+    const float *c = tab + idx * FSAMD_G_STRIDE;
     float r = c[FSAMD_G_DEG];
 #pragma unroll
     for (int k = FSAMD_G_DEG - 1; k >= 0; --k) r = __builtin_fmaf(r, t, c[k]);
@@ -94,47 +106,47 @@ __device__ inline float exp_neg_df(float p, float d)
     return __builtin_ldexpf(__builtin_amdgcn_exp2f(-f), -(int)n);
 }
 
-// H(a, u) for |u| < 7; (uh, ul) = |u| as a double-float pair, uh >= 0.
-__device__ inline float voigt_core(const VoigtA &v, float uh, float ul, const float *gtab)
+// H(a, u) with u given in FP64 (the sample's offset from the line centre in
+// units of b), any u.  The table `tab` is 16-byte aligned.
+__device__ inline float voigt32(const VoigtA &v, double u64, const float *tab)
 {
-    const float p = uh * uh;
-    const float d = __builtin_fmaf(2.0f * uh, ul, __builtin_fmaf(uh, uh, -p));
-    const float E = exp_neg_df(p, d);
-    const float c = 2.0f * v.a * uh;
+    static_assert(FSAMD_G_DEG == 6 && FSAMD_G_STRIDE == 8, "voigt32 evaluates 7 coefficients per row");
+    const float uh = (float)__builtin_fabs(u64);
+    // exp(a^2 - u^2) = 2^-y, y = (u^2 - a^2) log2(e) in FP64: 2^-frac(y) in FP32
+    // and the integer part by ldexp (0 once y passes the FP32 range)
+    const double u2 = u64 * u64;
+    const double y = __builtin_fmin(__builtin_fma(u2, kLog2E, -v.a2_log2e), 512.0);
+    const double n = __builtin_rint(y);
+    const float E = __builtin_ldexpf(__builtin_amdgcn_exp2f(-(float)(y - n)), -(int)n);
+    const float c = v.two_a * __builtin_amdgcn_fmed3f(uh, 0.0f, 12.0f);
     const float c2 = c * c;
     float cs = __builtin_fmaf(c2, -1.0f / 40320.0f, 1.0f / 720.0f);
     cs = __builtin_fmaf(c2, -cs, 1.0f / 24.0f);
     cs = __builtin_fmaf(c2, -cs, 0.5f);
     cs = __builtin_fmaf(c2, -cs, 1.0f);
-    const float re_exp = E * v.ea2 * cs;      // Re exp(-z^2)
-    const float G = g_of_u(uh, gtab);
-    const float D3 = __builtin_fmaf(__builtin_fmaf(4.0f, p, -6.0f), G, 2.0f);
-    return __builtin_fmaf(-v.two_a_sqpi, __builtin_fmaf(-v.a2_6, D3, G), re_exp);
-}
-
-// H(a, u) for |u| >= 7 (uh = |u|).
-__device__ inline float voigt_wing(const VoigtA &v, float uh)
-{
-    const float x = __builtin_amdgcn_rcpf(uh * uh);
-    const float P[9] = FSAMD_WING_P;
-    const float Q[3] = FSAMD_WING_Q;
-    float s = P[8];
-#pragma unroll
-    for (int k = 7; k >= 0; --k) s = __builtin_fmaf(s, x, P[k]);
-    float q = __builtin_fmaf(Q[2], x, Q[1]);
-    q = __builtin_fmaf(q, x, Q[0]);
-    return v.a_sqpi * x * __builtin_fmaf(-v.a2 * x, q, s);
-}
-
-// H(a, u) with u given in FP64 (the sample's offset from the line centre in
-// units of b); the FP32 profile math starts from its double-float split.
-__device__ inline float voigt32(const VoigtA &v, double u64, const float *gtab)
-{
-    const double au = u64 < 0 ? -u64 : u64;
-    const float uh = (float)au;
-    if (uh >= kVoigtCoreMax) return voigt_wing(v, uh);
-    const float ul = (float)(au - (double)uh);
-    return voigt_core(v, uh, ul, gtab);
+    const float re_exp = E * cs;
+    // G: table piece (|u| < 7) or x R(x) (wing row), selected per lane
+    const float p = (float)u2;
+    const float x = __builtin_amdgcn_rcpf(p);
+    const bool core = uh < kVoigtCoreMax;
+    const float q = uh * (1.0f / FSAMD_G_WIDTH);
+    const float t = __builtin_fmaf(__builtin_amdgcn_fractf(q), 2.0f, -1.0f);
+    const int idx = core ? (int)q : FSAMD_G_NPIECE;
+    const float var = core ? t : x;
+    const float4 lo = *(const float4 *)(tab + idx * FSAMD_G_STRIDE);
+    const float4 hi = *(const float4 *)(tab + idx * FSAMD_G_STRIDE + 4);
+    float r = __builtin_fmaf(hi.z, var, hi.y);
+    r = __builtin_fmaf(r, var, hi.x);
+    r = __builtin_fmaf(r, var, lo.w);
+    r = __builtin_fmaf(r, var, lo.z);
+    r = __builtin_fmaf(r, var, lo.y);
+    r = __builtin_fmaf(r, var, lo.x);
+    const float G = core ? r : x * r;
+    // a^3 term: Z = -D3/3 with D3 = D''' = (4u^2 - 6) G + 2, or x^2 Q(x) in the wing
+    const float Zc = (-1.0f / 3.0f) * __builtin_fmaf(__builtin_fmaf(4.0f, p, -6.0f), G, 2.0f);
+    const float Zw = x * x * __builtin_fmaf(__builtin_fmaf(26.25f, x, 5.0f), x, 1.0f);
+    const float Z = core ? Zc : Zw;
+    return __builtin_fmaf(-v.a3_sqpi, Z, __builtin_fmaf(-v.two_a_sqpi, G, re_exp));
 }
 
 // ---------------------------------------------------------------------------

@@ -260,14 +260,17 @@ def colden(plan: Plan, nbins: int, kernel: int, velfac: float, dens: Sequence[to
 
 
 def last_stats():
-    buf = (C.c_longlong * 6)()
-    N.check(N.lib().fsamd_last_stats(buf))
+    buf = (C.c_longlong * 8)()
+    N.check(N.lib().fsamd_last_stats(buf, 8))
     return {"pairs": buf[0], "pixels": buf[1], "fp64_pixels": buf[2], "fp64_pairs": buf[3],
-            "voigt_samples": buf[4], "evaluated_pixels": buf[5], "gpu_ms": float(N.lib().fsamd_last_ms())}
+            "voigt_samples": buf[4], "evaluated_pixels": buf[5], "kernel_sums": buf[6], "far_pixels": buf[7],
+            "gpu_ms": float(N.lib().fsamd_last_ms())}
 
 
-def set_stats(enable: bool):
-    N.check(N.lib().fsamd_set_stats(1 if enable else 0))
+def set_stats(enable: bool, timing_only: bool = False):
+    """Collect the counters and GPU time of each tau call (last_stats), or
+    with timing_only just the time (the kernels then run without counting)."""
+    N.check(N.lib().fsamd_set_stats(0 if not enable else (2 if timing_only else 1)))
 
 
 # ---------------------------------------------------------------------------

@@ -114,20 +114,24 @@ int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double bo
                                const float *h, long long npart, const int *axis,
                                const double *cofm, int nlos, double *out, void *stream);
 
-/* Counters of the last fsamd_tau call on this thread (host), six entries:
- * pairs deposited, pixels deposited, pixels re-evaluated in FP64 at the tail
- * cut, pairs evaluated entirely in FP64, Voigt samples of the deposited
- * pixels (7 x sub-samples, as the reference counts them), pixels evaluated
- * (deposited or not); and the GPU time of its launches in ms (hipEvents on
- * the call's stream).  Collected only after fsamd_set_stats(1), which makes
- * fsamd_tau synchronize the stream. */
-int fsamd_set_stats(int enable);
-int fsamd_last_stats(long long *out6);
+/* Counters of the last fsamd_tau call on this thread (host), FSAMD_NSTATS
+ * entries: pairs deposited, pixels deposited, pixels re-evaluated in FP64 at
+ * the tail cut, pairs evaluated entirely in FP64, Voigt samples of the
+ * deposited pixels (7 x sub-samples, as the reference counts them), pixels
+ * evaluated (deposited or not), 7-sample kernel sums evaluated (one per
+ * pixel sub-sample), pixels evaluated by the far-wing expansion; fsamd_last_stats
+ * copies the first min(n, FSAMD_NSTATS).  And the GPU time of its launches in ms (hipEvents on
+ * the call's stream).  fsamd_set_stats(flags): bit 0 collects the counters
+ * and the time, bit 1 the time only (the kernels then run without their
+ * counting code); either makes fsamd_tau synchronize the stream. */
+int fsamd_set_stats(int flags);
+#define FSAMD_NSTATS 8
+int fsamd_last_stats(long long *out, int n);
 double fsamd_last_ms(void);
 
 /* Autotuner (no counterpart in the reference): times the batched tau launch
- * for lane-group widths 16, 32 and 64 on every 4th sightline of the plan's
- * heaviest-first order (all of them below 4096 sightlines) and keeps the
+ * for lane-group widths 16, 32 and 64 on every k-th sightline of the plan's
+ * heaviest-first order (k = nlos / 16384, all sightlines below 32768) and keeps the
  * fastest for later fsamd_tau calls with the same nbins, line count and
  * kernel.  Writes partial results into the lines' outputs.  ms3 (optional,
  * host) receives the three times. */

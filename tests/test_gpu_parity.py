@@ -188,3 +188,33 @@ def test_cold_gas_uses_fp64_profile(oracle):
         SP.set_stats(False)
     assert st["fp64_pairs"] > 0
     assert_tau_close(got, oracle.interpolate(*args), "cold gas")
+
+
+def test_stats_counters_consistent():
+    """The counting kernels deposit the same spectra as the plain ones (up to
+    the order of the FP64 additions), and
+    their counters add up: every evaluated pixel is either a far-wing pixel or
+    one or more 7-sample kernel sums, and at least one per deposited pixel."""
+    w = synth.config(1, scale=1 / 64, nlos=16)
+    plan = plan_for(w)
+    args = (plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), w.lines,
+            [dev(w.dens_for(ln)) for ln in w.lines], w.tautail)
+    base = [o.cpu().numpy() for o in SP.tau(*args)]
+    for timing_only in (False, True):
+        SP.set_stats(True, timing_only=timing_only)
+        try:
+            outs = [o.cpu().numpy() for o in SP.tau(*args)]
+            st = SP.last_stats()
+        finally:
+            SP.set_stats(False)
+        for a, b in zip(outs, base):
+            # FP64 LDS atomics add in launch-dependent order
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-300)
+        assert st["gpu_ms"] > 0
+        if timing_only:
+            continue
+        assert st["pairs"] > 0 and st["pixels"] > 0
+        assert st["evaluated_pixels"] >= st["pixels"]
+        assert st["far_pixels"] < st["evaluated_pixels"]
+        assert st["kernel_sums"] >= st["evaluated_pixels"] - st["far_pixels"]
+        assert st["voigt_samples"] >= 7 * (st["pixels"] - st["far_pixels"])
