@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-config", type=int, default=3,
+                    help="workload of the forward vs forward+adjoint leg (3: 128^3, 1024 x 1024, H I 1215)")
+    ap.add_argument("--colden", action="store_true",
+                    help="also time the column density of the first line's density (binning included)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -55,44 +59,50 @@ def copy_probe(mib):
 
 def cpu_baseline(w, budget_s):
     """The reference C++ (oracle/_ref, OpenMP, all host threads) on every k-th
-    sightline of the same workload with the full particle set: first line only,
-    grown until about budget_s of CPU time.  Falls back to the oracle's C
-    restatement where the reference library is not built."""
+    sightline of the same workload with the full particle set, one
+    _Particle_Interpolate call per line as the reference makes them
+    (spectra.py:673), the sample grown until about budget_s of CPU time.
+    Falls back to the oracle's C restatement where the reference library is
+    not built."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     kind = "reference" if O.ref_available() else "port"
     lib = O.load_ref() if kind == "reference" else O.load_oracle()
     cores = lib.max_threads()
-    line = w.lines[0]
     nsel = 4
     t = 0.0
     while True:
         k = max(1, w.nlos // nsel)
         sel = np.arange(0, w.nlos, k)[:nsel]
-        args = list(w.interp_args(line, True))
-        args[16], args[17] = w.axis[sel], w.cofm[sel]
-        t0 = time.perf_counter()
-        lib.interpolate(*args)
-        t = time.perf_counter() - t0
+        t = 0.0
+        for line in w.lines:
+            args = list(w.interp_args(line, True))
+            args[16], args[17] = w.axis[sel], w.cofm[sel]
+            t0 = time.perf_counter()
+            lib.interpolate(*args)
+            t += time.perf_counter() - t0
         if t >= budget_s / 3 or nsel >= w.nlos:
             break
         nsel = min(w.nlos, max(nsel * 2, int(nsel * budget_s / 3 / max(t, 1e-3))))
-    value = len(sel) * w.nbins / t
+    value = len(w.lines) * len(sel) * w.nbins / t
     return {"value": value, "unit": "sightline-pixels/s", "cores": cores, "kind": kind,
-            "sample": "%s line %s, %d of %d sightlines (every %d-th), all %d particles, index table + deposit, "
-                      "%.1f s" % ("reference C++ -O2 -ffast-math OpenMP" if kind == "reference" else "C restatement",
-                                  line.name, len(sel), w.nlos, k, w.npart, t)}
+            "sample": "%s, lines %s (one call each), %d of %d sightlines (every %d-th), all %d particles, "
+                      "index table + deposit, %.1f s" % (
+                          "reference C++ -O2 -ffast-math OpenMP" if kind == "reference" else "C restatement",
+                          "+".join(ln.name for ln in w.lines), len(sel), w.nlos, k, w.npart, t)}
 
 
-def adjoint_leg(steps):
-    """Forward and forward+adjoint on the differentiable workload (128^3
-    particles, 1024 sightlines x 1024 pixels, H I 1215): loss
-    sum (exp(-tau) - F)^2, F from the same field with T scaled by 1.1."""
+def adjoint_leg(steps, config=3, w=None):
+    """Forward and forward+adjoint of the first line: loss
+    sum (exp(-tau) - F)^2, F from the same field with T scaled by 1.1
+    (by default the differentiable workload, 128^3 particles, 1024 sightlines
+    x 1024 pixels, H I 1215)."""
     import numpy as np
     import torch
     from fake_spectra_amd import synth, spectra_priv as SP
-    w = synth.config(3)
+    if w is None:
+        w = synth.config(config)
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     pos, hh, ax, cofm = dev(w.pos), dev(w.hh), dev(w.axis), dev(w.cofm)
     vel, temp, dens = dev(w.vel), dev(w.temp), dev(w.dens["H1"])
@@ -126,11 +136,35 @@ def adjoint_leg(steps):
         torch.cuda.synchronize()
         res[name] = (time.perf_counter() - t0) / steps
     units = w.nlos * w.nbins
-    return {"workload": "128^3 particles, 1024 sightlines x 1024 pixels, H I 1215, loss sum (exp(-tau)-F)^2",
+    return {"workload": "c%d: %d^3 particles, %d sightlines x %d pixels, %s, kernel %s, loss sum (exp(-tau)-F)^2" % (
+                config, round(w.npart ** (1 / 3)), w.nlos, w.nbins, line.name, "cubic" if w.kernel == 1 else "tophat"),
             "forward_ms": 1e3 * res["forward"], "forward_adjoint_ms": 1e3 * res["forward_adjoint"],
             "forward_value": units / res["forward"], "forward_adjoint_value": units / res["forward_adjoint"],
             "adjoint_over_forward": res["forward_adjoint"] / res["forward"], "unit": "sightline-pixels/s",
             "note": "binning included in both; fwd+adjoint records windows and runs the adjoint kernel"}
+
+
+def colden_leg(w, pos, hh, ax, cofm, dens, steps):
+    """Column density of one density field on the workload's sightlines,
+    binning included (the _Particle_Interpolate colden call)."""
+    import torch
+    from fake_spectra_amd import spectra_priv as SP
+    out = [torch.empty((w.nlos, w.nbins), dtype=torch.float64, device="cuda")]
+
+    def run():
+        plan = SP.Plan(w.box, pos, hh, ax, cofm)
+        SP.colden(plan, w.nbins, w.kernel, w.velfac, [dens], out=out)
+        plan.close()
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {"value": w.nlos * w.nbins / dt, "unit": "sightline-pixels/s", "ms_per_call": 1e3 * dt,
+            "note": "binning + batched colden launch"}
 
 
 def main():
@@ -264,6 +298,8 @@ def main():
     out["autotune"] = {"group_width": best_g, "sample_ms": {str(k): v for k, v in tune_ms.items()}}
     if not args.no_plain:
         plan = SP.Plan(w.box, pos, hh, ax, cofm)
+        # one untimed call builds the per-sightline work items
+        SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail, out=outs, mode=N.MODE_PLAIN)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail, out=outs, mode=N.MODE_PLAIN)
@@ -274,8 +310,11 @@ def main():
                         "batched_over_plain": tplain / tbatch,
                         "note": "deposit only, same binning; one launch per sightline and line"}
         plan.close()
+    if args.colden:
+        out["colden"] = colden_leg(w, pos, hh, ax, cofm, dens[0], args.steps)
     if not args.no_adjoint:
-        out["adjoint"] = adjoint_leg(max(2, args.steps))
+        same = args.adjoint_config == args.config
+        out["adjoint"] = adjoint_leg(max(2, args.steps), args.adjoint_config, w if same else None)
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_seconds)
         out["cpu_baseline"] = cb

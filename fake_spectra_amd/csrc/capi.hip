@@ -170,8 +170,8 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
 {
     if (!P) return fail(FSAMD_ERR_ARG, "plan is NULL");
     if (nbins < 1) return fail(FSAMD_ERR_ARG, "nbins must be positive");
-    if (kernel != kTopHat && kernel != kCubicSpline)
-        return fail(FSAMD_ERR_UNSUPPORTED, "kernel must be 0 (tophat) or 1 (cubic spline)");
+    if (kernel != kTopHat && kernel != kCubicSpline && kernel != kSphQuintic)
+        return fail(FSAMD_ERR_UNSUPPORTED, "kernel must be 0 (tophat), 1 (cubic spline) or 3 (quintic spline)");
     if (nlines < 1 || !lines) return fail(FSAMD_ERR_ARG, "at least one line is required");
     if (mode != FSAMD_MODE_BATCHED && mode != FSAMD_MODE_PLAIN) return fail(FSAMD_ERR_ARG, "unknown mode");
     if ((long long)P->nlos * nbins >= (1ll << 62)) return fail(FSAMD_ERR_ARG, "output too large");

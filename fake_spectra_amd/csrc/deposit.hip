@@ -61,9 +61,25 @@ __device__ inline double sph_cubic_kernel(double q)
     return norm * (2 * (r * r * r));
 }
 
+// singleabs.h:31-42: quintic spline of support 1 as a sum of truncated fifth
+// powers, 9/(40 pi) sum_k c_k max(0, r_k - 3q)^5, (r, c) = (3, 1), (2, -6), (1, 15)
+__device__ inline double sph_quintic_kernel(double q)
+{
+    const double norm = 9. / 40 / kPi;
+    if (q >= 1) return 0;
+    const double t3 = 3. - 3. * q, t2 = 2. - 3. * q, t1 = 1. - 3. * q;
+    double f = t3 * t3 * t3 * t3 * t3;
+    if (t2 > 0) f -= 6. * (t2 * t2 * t2 * t2 * t2);
+    if (t1 > 0) f += 15. * (t1 * t1 * t1 * t1 * t1);
+    return norm * f;
+}
+
+// Kernel weight of tau_kern_inner (singleabs.h:158-163)
 __device__ inline double kernel_weight(int kernel, double q)
 {
-    return kernel == kCubicSpline ? sph_cubic_kernel(q) : 3. / 4. / kPi;
+    if (kernel == kCubicSpline) return sph_cubic_kernel(q);
+    if (kernel == kSphQuintic) return sph_quintic_kernel(q);
+    return 3. / 4. / kPi;
 }
 
 __device__ inline LineDesc line_of(const DepositArgs &A, int k) { return A.lines[k]; }
@@ -148,6 +164,9 @@ struct PairTau {
     int fp64;
     int np;          // sub-samples per pixel (1 = midpoint)
     int wing_ready;  // Cw computed
+    double dv;       // sub-sample spacing bintov / (np - 1)
+    double inv_nm1;  // 1 / (np - 1)
+    float inv_nm1f;
 };
 
 __device__ inline void wave_sync()
@@ -309,7 +328,7 @@ __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
 // not reach the sightline.  Every lane of the wave calls it.
 template <int kG>
 __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const LosInfo &los, int p,
-                                  PairTau &P, int lane)
+                                  PairTau &P, int lane, int max_sums = 1 << 30)
 {
     const float *pp = &A.pos[3 * (size_t)p];
     const double dr2 = pair_dr2(los, pp, A.box);
@@ -332,9 +351,17 @@ __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const
     const double bintov = A.vbox / A.nbins;
     const double amp = L.sigma_a / sqrt(kPi) * (kLight / 1e5 / b);
     const double dvk = 2. * vhigh / kNGrid;
-    const int fp64 = !(aa <= (double)kA32Max);
     int np = 1;
     if (!(bintov < b / 2.)) np = 2 * ceil(bintov / (b / 2.) / 2) + 1.;
+    // The reference sizes the sub-sample grid per pixel from vhigh - vlow,
+    // which equals bintov up to the rounding of vlow + bintov (~1e-13
+    // relative); the walk uses one grid per pair, so a pair whose grid could
+    // differ between pixels (bintov/b at a switch point to 1e-12) goes to the
+    // FP64 path, which follows the reference pixel by pixel.
+    const double xs = bintov / (b / 2.) / 2;
+    const bool grid_edge = fabs(bintov - b / 2.) <= 1e-12 * bintov || (np > 1 && fabs(xs - rint(xs)) <= 1e-12 * xs);
+    // pixels wider than max_sums sub-samples do not fit one walk step
+    const int fp64 = !(aa <= (double)kA32Max) || grid_edge || np - 1 > max_sums - 1;
     const double dmax = (vhigh - dvk) * inv_b + (np > 1 ? 0.5 * bintov * inv_b : 0.0);
     const double ufar = fmax(7.0 + dmax, 3.0 * dmax);
     wave_sync();   // the previous pair's constants are no longer read
@@ -364,6 +391,9 @@ __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const
         P.ufar = ufar;
         P.ufar2 = (float)(ufar * ufar);
         P.wing_ready = 0;
+        P.dv = np > 1 ? bintov / (np - 1) : 0.0;
+        P.inv_nm1 = np > 1 ? 1.0 / (np - 1) : 1.0;
+        P.inv_nm1f = (float)P.inv_nm1;
     }
     wave_sync();
     return true;
@@ -478,6 +508,139 @@ __device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double
     return false;
 }
 
+// Sub-sample grid of one walk step in the main kernel.  A step evaluates one
+// 7-sample kernel sum per lane and round, on the sub-sample grid of
+// tau_kern_outer (singleabs.h:104-126) laid out across the lanes: adjacent
+// pixels share their boundary sub-sample, so a step of p pixels with np
+// sub-samples each needs p (np - 1) + 1 sums; midpoint pixels (np = 1) need
+// one each.  Pairs whose grid exceeds kRoundsMax rounds take the FP64 path.
+constexpr int kRoundsMax = 4;
+
+struct StepShape {
+    int nm1;      // np - 1 (0: midpoint)
+    int rounds;   // lane rounds per step
+    int pstep;    // pixels per step
+};
+
+__device__ inline StepShape step_shape(int np, int kg)
+{
+    StepShape sh;
+    sh.nm1 = np - 1;
+    if (sh.nm1 == 0) {
+        sh.rounds = 1;
+        sh.pstep = kg;
+    } else if (sh.nm1 < kg) {
+        sh.rounds = 1;
+        sh.pstep = (kg - 1) / sh.nm1;
+    } else {
+        sh.rounds = min(kRoundsMax, (2 * sh.nm1 + kg) / kg);
+        sh.pstep = (kg * sh.rounds - 1) / sh.nm1;
+    }
+    return sh;
+}
+
+// One step of a walk in the main kernel (lane glane of a kG-lane group):
+// the next pixels z0 + dir*(c + p), p < pstep, of the walk.  Pixels whose
+// centre lies in the far wing take the moment expansion (a step is a far
+// step when its first pixel is: |U| only grows along a walk).  `sbuf` is the
+// group's LDS scratch of kG * kRoundsMax floats.  Returns true when the walk
+// is over (tail cut, end of the window, or the decision deferred to the
+// fix-up kernel); `adv` is the number of pixels the step covered.
+template <int kG, bool kLds, bool kStats>
+__device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *sbuf, double *row,
+                                 const float *gtab, int z0, int dir, int count, int c, int lane, LaneStats &st,
+                                 long long k, int item, bool active, int &walked, int &adv)
+{
+    const int glane = lane % kG;
+    const double bintov = A.vbox / A.nbins;
+    const double tt = A.tautail;
+    const double U0 = ((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
+    const bool far = fabs(U0) >= P.ufar;
+    const StepShape sh = step_shape(far ? 1 : P.np, kG);
+    const int pcount = active ? min(sh.pstep, count - c) : 0;
+    wave_sync();   // the previous step's reads of sbuf are done
+    if (active && !far) {
+        const int nsub = sh.nm1 == 0 ? pcount : pcount * sh.nm1 + 1;
+        const int zlo = dir > 0 ? z0 + c : z0 - c - (pcount - 1);
+        for (int r = 0; r < sh.rounds; ++r) {
+            const int m = glane + r * kG;
+            if (m < nsub) {
+                double v;
+                if (sh.nm1 == 0) {
+                    const double vlow = (z0 + dir * (c + m)) * bintov - P.vel;   // absorption.cpp:252
+                    v = ((vlow + bintov) + vlow) / 2.;                           // singleabs.h:113
+                } else {
+                    // sub-sample kk of pixel zlo + j, in increasing velocity
+                    const int j = (int)(((float)m + 0.5f) * P.inv_nm1f);
+                    const int kk = m - j * sh.nm1;
+                    const double vlow = (zlo + j) * bintov - P.vel;
+                    v = kk == 0 ? vlow : kk * P.dv + vlow;                       // singleabs.h:118-121
+                }
+                sbuf[m] = (float)inner32(P, v, gtab);
+                if (kStats) st.sums++;
+            }
+        }
+    }
+    wave_sync();
+    const bool valid = glane < pcount;
+    const int z = z0 + dir * (c + glane);
+    const double vlow = z * bintov - P.vel;
+    double t = 0;
+    bool guard = false;
+    if (valid) {
+        if (far) {
+            const double U = (vlow + 0.5 * bintov) * P.inv_b;
+            const float Y = P.ufar2 * __builtin_amdgcn_rcpf((float)(U * U));
+            float S = P.Cw[kWingTerms - 1];
+#pragma unroll
+            for (int q = kWingTerms - 2; q >= 0; --q) S = __builtin_fmaf(S, Y, P.Cw[q]);
+            t = P.scale * (double)(S * Y);
+        } else if (sh.nm1 == 0) {
+            t = P.scale * (double)sbuf[glane];
+        } else {
+            // trapezoid over the pixel's sub-samples (singleabs.h:116-125)
+            const int j = dir > 0 ? glane : pcount - 1 - glane;
+            const float *sp = sbuf + j * sh.nm1;
+            double tot = 0.5 * (double)sp[0];
+            for (int kk = 1; kk < sh.nm1; ++kk) tot += (double)sp[kk];
+            tot += 0.5 * (double)sp[sh.nm1];
+            t = P.scale * (tot * P.inv_nm1);
+        }
+        if (kStats) {
+            st.evaluated++;
+            st.far += far;
+        }
+        guard = fabs(t - tt) <= kGuard * tt;
+    }
+    const unsigned long long mb = group_bits<kG>(__ballot(valid && !guard && t < tt), lane);
+    const unsigned long long mg = group_bits<kG>(__ballot(guard), lane);
+    const unsigned long long mall = mb | mg;
+    const int first = mall ? __builtin_ctzll(mall) : kG;
+    if (valid && (glane < first || (glane == first && ((mb >> glane) & 1ull)))) {
+        row_add<kLds>(&row[wrap_pixel(z, A.nbins)], t);
+        if (kStats) {
+            st.pixels++;
+            st.samples += pixel_samples(P, vlow, bintov);
+        }
+    }
+    adv = pcount;
+    if (first < kG && ((mg >> first) & 1ull)) {
+        // the FP32 value of the deciding pixel is too close to tautail
+        if (active && glane == 0) push_fixup(A, k, item, z0 + dir * (c + first), count - (c + first), dir);
+        walked = -1;
+        return true;
+    }
+    if (mb != 0) {
+        walked = c + first + 1;
+        return true;
+    }
+    if (c + pcount >= count) {
+        walked = count;
+        return true;
+    }
+    return false;
+}
+
 // Record the pixels walked in one direction of pair k (for the adjoint).
 __device__ inline void record_window(const DepositArgs &A, int line, long long k, int dir, int len)
 {
@@ -525,6 +688,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     extern __shared__ double smem[];
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kGroups];
+    __shared__ float sub_sums[kThreads * kRoundsMax];
     __shared__ int s_item;
     __shared__ unsigned long long s_next;
     for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
@@ -533,6 +697,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int gid = threadIdx.x / kG;
     const int gleader = (lane / kG) * kG;
     PairTau &P = pslot[gid];
+    float *sbuf = sub_sums + gid * kG * kRoundsMax;
     const double bintov = A.vbox / A.nbins;
     const int half = A.nbins / 2;
     LaneStats st;
@@ -569,7 +734,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 kk = (unsigned long long)__shfl((long long)kk, gleader, 64);
                 if ((long long)kk >= w.p1) {
                     dir = 2;
-                } else if (setup_pair<kG>(A, L, los, A.pair_part[kk], P, glane)) {
+                } else if (setup_pair<kG>(A, L, los, A.pair_part[kk], P, glane, kG * kRoundsMax)) {
                     k = (long long)kk;
                     if (kStats) st.pairs += (glane == 0);
                     if (P.fp64) {
@@ -584,18 +749,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             }
             const bool walking = (dir == 1 || dir == -1);
             if (!__ballot(dir != 2)) break;
-            if (walking && !P.wing_ready && step_needs_wing(P, bintov, z0, dir, c, kG, half)) {
+            if (walking && !P.wing_ready && step_needs_wing(P, bintov, z0, dir, c, 1, half)) {
                 wing_setup<kG>(P, bintov, glane);
                 if (glane == 0) P.wing_ready = 1;
                 wave_sync();
             }
-            int walked = 0;
-            const bool done = walk_chunk<kG, kWalkMain, kLds, kStats>(A, P, row, gtab, z0, dir, half, c, lane, st, k, gitem,
-                                                              walking, walked);
+            int walked = 0, adv = 0;
+            const bool done = walk_step<kG, kLds, kStats>(A, P, sbuf, row, gtab, z0, dir, half, c, lane, st, k, gitem,
+                                                          walking, walked, adv);
             if (walking) {
                 if (done && walked >= 0 && glane == 0) record_window(A, w.line, k, dir, walked);
                 if (!done) {
-                    c += kG;
+                    c += adv;
                 } else if (dir == 1) {
                     dir = -1;
                     z0 = P.zmax - 1;
@@ -876,21 +1041,21 @@ int adjoint_blocks_per_cu(int gsize, int nbins)
 // ---------------------------------------------------------------------------
 // Column density
 
-// absorption.cpp:53-74 (cubic) / 109-115 (tophat)
+// absorption.cpp:53-74 (cubic), 76-97 (quintic) / 109-115 (tophat)
 __device__ inline double kern_frac(int kernel, double zlow, double zhigh, double smooth, double dr2,
                                    double zrange)
 {
     zlow = fmax(zlow, -zrange);
     zhigh = fmin(zhigh, zrange);
-    if (kernel != kCubicSpline) return 3. / 4. / kPi * fmax(0., zhigh - zlow);
+    if (kernel != kCubicSpline && kernel != kSphQuintic) return 3. / 4. / kPi * fmax(0., zhigh - zlow);
     if (zlow > zhigh) return 0;
-    double total = sph_cubic_kernel(sqrt(dr2 + zlow * zlow) / smooth) / 2.;
+    double total = kernel_weight(kernel, sqrt(dr2 + zlow * zlow) / smooth) / 2.;
     const double dz = (zhigh - zlow) / kNGrid;
     for (int i = 1; i < kNGrid; ++i) {
         const double zz = i * dz + zlow;
-        total += sph_cubic_kernel(sqrt(dr2 + zz * zz) / smooth);
+        total += kernel_weight(kernel, sqrt(dr2 + zz * zz) / smooth);
     }
-    total += sph_cubic_kernel(sqrt(dr2 + zhigh * zhigh) / smooth) / 2.;
+    total += kernel_weight(kernel, sqrt(dr2 + zhigh * zhigh) / smooth) / 2.;
     return dz * total;
 }
 

@@ -17,6 +17,7 @@ constexpr double kPi = 3.14159265358979323846;
 
 constexpr int kTopHat = 0;       // singleabs.h:9
 constexpr int kCubicSpline = 1;  // singleabs.h:10
+constexpr int kSphQuintic = 3;   // singleabs.h:12
 constexpr int kNGrid = 8;        // singleabs.h:8
 
 void set_error(const std::string &msg);

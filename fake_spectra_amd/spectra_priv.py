@@ -328,8 +328,8 @@ def _Particle_Interpolate(*args):
         raise ValueError("pos must have shape (N, 3)")
     if compute_tau and (vel.size < 3 * npart or temp.size < npart):
         raise ValueError("vel and temp must have one entry per particle when compute_tau is set")
-    if kernel not in (0, 1):
-        raise ValueError("kernel %d is not supported: 0 (tophat) and 1 (cubic SPH) are" % kernel)
+    if kernel not in (0, 1, 3):
+        raise ValueError("kernel %d is not supported: 0 (tophat), 1 (cubic SPH) and 3 (quintic SPH) are" % kernel)
     if numlos and (axis.min() < 1 or axis.max() > 3):
         raise ValueError("axis values must be 1, 2 or 3")
     if nbins < 1:

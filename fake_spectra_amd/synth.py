@@ -78,6 +78,7 @@ class Workload:
     temp: np.ndarray           # float32 [N], K
     hh: np.ndarray             # float32 [N], kernel support radius, kpc/h
     overdensity: np.ndarray    # float32 [N], 1+delta
+    neutral_fraction: np.ndarray   # float32 [N], x_HI
     axis: np.ndarray           # int32 [L], 1..3
     cofm: np.ndarray           # float64 [L,3]
     lines: List[Line]
@@ -198,6 +199,7 @@ def make_gas(name, n_side, box, sigma_ln, nbins, kernel, lines, axis, cofm, seed
     w = Workload(name=name, box=float(box), nbins=int(nbins), kernel=int(kernel), cosmo=cosmo,
                  pos=pos, vel=vel, temp=temp32, hh=hh.astype(np.float32),
                  overdensity=one_plus_delta.astype(np.float32),
+                 neutral_fraction=x_hi.astype(np.float32),
                  axis=np.ascontiguousarray(axis, dtype=np.int32),
                  cofm=np.ascontiguousarray(cofm, dtype=np.float64), lines=list(lines), dens=dens)
     return w

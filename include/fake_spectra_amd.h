@@ -68,7 +68,8 @@ typedef struct fsamd_line {
 
 /* Optical depth of nlines lines (ParticleInterp::compute_tau,
  * part_int.cpp:20-51, with LineAbsorption::add_tau_particle,
- * absorption.cpp:212-279).  kernel: 0 tophat, 1 cubic spline.
+ * absorption.cpp:212-279).  kernel: 0 tophat, 1 cubic spline, 3 quintic spline
+ * (singleabs.h:9-12; the Voronoi mesh, 2, is not supported).
  * mode: FSAMD_MODE_BATCHED or FSAMD_MODE_PLAIN. */
 int fsamd_tau(fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail,
               const float *pos, const float *vel, const float *temp, const float *h,

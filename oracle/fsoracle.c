@@ -11,6 +11,9 @@
  *                         table (Faddeeva.cpp:1919-2106), test.cpp:284-302
  *                         known answers, and the compiled reference.
  *   fso_sph_cubic_kernel  closed form + test.cpp:37-46
+ *   fso_sph_quintic_kernel closed forms (normalisation 4 pi int K q^2 dq = 1,
+ *                         value and continuity at q = 0, 1/3, 2/3, 1) + the
+ *                         compiled reference
  *   fso_kern_frac         test.cpp:48-79 + exact integral of the kernel
  *   fso_tau_kern_outer    test.cpp:304-332 + compiled reference
  *   fso_add_*_particle    test.cpp:83-151, 339-404 + compiled reference
@@ -38,6 +41,7 @@
 #define NGRID 8                 /* singleabs.h:8 */
 #define TOP_HAT_KERNEL 0        /* singleabs.h:9 */
 #define SPH_CUBIC_SPLINE 1      /* singleabs.h:10 */
+#define SPH_QUINTIC_SPLINE 3    /* singleabs.h:12 */
 
 /* absorption.cpp:21-33, absorption.h:4 */
 #define SIGMA_T 6.652458558e-25
@@ -183,20 +187,42 @@ double fso_sph_cubic_kernel(double q)
     return norm * (2 * pow(1. - q, 3));
 }
 
-/* absorption.cpp:53-74 (cubic) and 109-115 (tophat) */
+/* singleabs.h:31-42: the quintic spline of support 1, written as a sum of
+   truncated fifth powers, K = 9/(40 pi) sum_k c_k max(0, r_k - 3q)^5 with
+   (r, c) = (3, 1), (2, -6), (1, 15) */
+double fso_sph_quintic_kernel(double q)
+{
+    const double norm = 9. / 40 / M_PI;
+    if (q >= 1) return 0;
+    const double t3 = 3. - 3. * q, t2 = 2. - 3. * q, t1 = 1. - 3. * q;
+    double f = t3 * t3 * t3 * t3 * t3;
+    if (t2 > 0) f -= 6. * (t2 * t2 * t2 * t2 * t2);
+    if (t1 > 0) f += 15. * (t1 * t1 * t1 * t1 * t1);
+    return norm * f;
+}
+
+double fso_kernel_weight(int kernel, double q)
+{
+    if (kernel == SPH_CUBIC_SPLINE) return fso_sph_cubic_kernel(q);
+    if (kernel == SPH_QUINTIC_SPLINE) return fso_sph_quintic_kernel(q);
+    return 3. / 4. / M_PI;
+}
+
+/* absorption.cpp:53-74 (cubic), 76-97 (quintic) and 109-115 (tophat):
+   8-interval trapezoid of the kernel over [zlow, zhigh] clamped to the support */
 double fso_kern_frac(int kernel, double zlow, double zhigh, double smooth, double dr2, double zrange)
 {
     zlow = fmax(zlow, -zrange);
     zhigh = fmin(zhigh, zrange);
-    if (kernel != SPH_CUBIC_SPLINE) return 3. / 4. / M_PI * fmax(0., zhigh - zlow);
+    if (kernel != SPH_CUBIC_SPLINE && kernel != SPH_QUINTIC_SPLINE) return 3. / 4. / M_PI * fmax(0., zhigh - zlow);
     if (zlow > zhigh) return 0;
-    double total = fso_sph_cubic_kernel(sqrt(dr2 + zlow * zlow) / smooth) / 2.;
+    double total = fso_kernel_weight(kernel, sqrt(dr2 + zlow * zlow) / smooth) / 2.;
     const double dz = (zhigh - zlow) / NGRID;
     for (int i = 1; i < NGRID; i++) {
         const double zz = i * dz + zlow;
-        total += fso_sph_cubic_kernel(sqrt(dr2 + zz * zz) / smooth);
+        total += fso_kernel_weight(kernel, sqrt(dr2 + zz * zz) / smooth);
     }
-    total += fso_sph_cubic_kernel(sqrt(dr2 + zhigh * zhigh) / smooth) / 2.;
+    total += fso_kernel_weight(kernel, sqrt(dr2 + zhigh * zhigh) / smooth) / 2.;
     return dz * total;
 }
 
@@ -229,11 +255,7 @@ static double tau_kern_inner(absorber_t *s, double vouter)
         const double vv = i * dv - s->vhigh;
         const double q = sqrt(s->vdr2 + vv * vv) / s->vsmooth;
         const double t0 = (vv - vouter) / s->btherm;
-        double tbin = fso_profile(t0, s->aa);
-        if (s->kernel == SPH_CUBIC_SPLINE)
-            tbin *= fso_sph_cubic_kernel(q);
-        else
-            tbin *= 3. / 4. / M_PI;
+        const double tbin = fso_profile(t0, s->aa) * fso_kernel_weight(s->kernel, q);
         total += tbin;
     }
     s->samples += NGRID - 1;

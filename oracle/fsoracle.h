@@ -17,6 +17,7 @@ void fso_w(double x, double y, double *wr, double *wi);
 double fso_profile(double u, double a);
 double fso_sph_cubic_kernel(double q);
 double fso_kern_frac(int kernel, double zlow, double zhigh, double smooth, double dr2, double zrange);
+double fso_sph_quintic_kernel(double q);
 double fso_tau_kern_outer(double btherm, double vdr2, double vsmooth, double aa, int kernel,
                           double vlow, double vhigh);
 void fso_add_tau_particle(double lambda, double gamma, double fosc, double amumass, double velfac,

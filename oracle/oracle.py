@@ -53,6 +53,8 @@ class _Lib:
         getattr(L, p + "profile").restype = _cd
         getattr(L, p + "sph_cubic_kernel").argtypes = [_cd]
         getattr(L, p + "sph_cubic_kernel").restype = _cd
+        getattr(L, p + "sph_quintic_kernel").argtypes = [_cd]
+        getattr(L, p + "sph_quintic_kernel").restype = _cd
         getattr(L, p + "kern_frac").argtypes = [_ci, _cd, _cd, _cd, _cd, _cd]
         getattr(L, p + "kern_frac").restype = _cd
         getattr(L, p + "tau_kern_outer").argtypes = [_cd, _cd, _cd, _cd, _ci, _cd, _cd]
@@ -141,6 +143,9 @@ class _Lib:
 
     def sph_cubic_kernel(self, q):
         return self._f("sph_cubic_kernel")(q)
+
+    def sph_quintic_kernel(self, q):
+        return self._f("sph_quintic_kernel")(q)
 
     def kern_frac(self, kernel, zlow, zhigh, smooth, dr2, zrange):
         return self._f("kern_frac")(kernel, zlow, zhigh, smooth, dr2, zrange)

@@ -29,6 +29,7 @@
 
 /* Defined with external linkage in absorption.cpp:109, not declared in a header. */
 double tophat_kern_frac(double zlow, double zhigh, const double smooth, const double dr2, const double zrange);
+double sph_quintic_kern_frac(double zlow, double zhigh, const double smooth, const double dr2, const double zrange);
 
 extern "C" {
 
@@ -123,10 +124,12 @@ void fsref_w(double re, double im, double *wr, double *wi)
 
 double fsref_profile(double u, double a) { return profile(u, a); }
 double fsref_sph_cubic_kernel(double q) { return sph_cubic_kernel(q); }
+double fsref_sph_quintic_kernel(double q) { return sph_quintic_kernel(q); }
 
 double fsref_kern_frac(int kernel, double zlow, double zhigh, double smooth, double dr2, double zrange)
 {
     if (kernel == SPH_CUBIC_SPLINE) return sph_cubic_kern_frac(zlow, zhigh, smooth, dr2, zrange);
+    if (kernel == SPH_QUINTIC_SPLINE) return sph_quintic_kern_frac(zlow, zhigh, smooth, dr2, zrange);
     return tophat_kern_frac(zlow, zhigh, smooth, dr2, zrange);
 }
 

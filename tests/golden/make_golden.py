@@ -11,7 +11,7 @@ Run in the container that holds /root/reference, after `make -C oracle ref`:
 * ref_cases.npz: small hand-built inputs (periodic wrap at every box face,
   duplicate sightlines, all three axes, supports wider than half the box,
   zero and tiny smoothing lengths, cold gas, Gamma = 0, odd and tiny nbins,
-  tophat and cubic kernels) with the reference's outputs.
+  tophat, cubic and quintic kernels) with the reference's outputs.
 """
 import json
 import os
@@ -126,7 +126,7 @@ def ref_cases(ref):
     velfac = 0.1115235417541224
     out = {}
     for name, c in crafted_cases().items():
-        for kern in (0, 1):
+        for kern in (0, 1, 3):
             for (ln, gam_scale, nb) in ((HI_1215, 1.0, 257), (HI_1215, 0.0, 64), (CIV_1548, 1.0, 300), (HI_1215, 1.0, 7)):
                 key = "%s_k%d_%s_g%g_n%d" % (name, kern, ln.name.replace(" ", ""), gam_scale, nb)
                 tau = ref.interpolate(1, nb, kern, c["box"], velfac, 0.25, ln.lambda_cm, ln.gamma * gam_scale,

@@ -93,9 +93,10 @@ def test_shape_errors_are_value_errors():
     with pytest.raises(ValueError):
         spectra_priv._Particle_Interpolate(*b)
     b = list(a)
-    b[2] = 3                    # quintic: not supported by this build
-    with pytest.raises(ValueError):
-        spectra_priv._Particle_Interpolate(*b)
+    for kernel in (2, 4, -1):   # Voronoi mesh and unknown ids
+        b[2] = kernel
+        with pytest.raises(ValueError):
+            spectra_priv._Particle_Interpolate(*b)
 
 
 def test_empty_inputs_give_zero_arrays():

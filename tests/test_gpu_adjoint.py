@@ -127,3 +127,47 @@ def test_adjoint_requires_recorded_windows():
     with pytest.raises(ValueError):
         SP.tau_adjoint(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), [HI_1215],
                        [dev(w.dens["H1"])], [g])
+
+
+def test_adjoint_neutral_fraction_through_torch():
+    """d loss / d x_HI through the density pipeline built in torch
+    (dens = x_HI * n_H rscale 0.76 / amu, spectra.py:593-615): autograd chains
+    the hand-written adjoint into torch ops; checked against central
+    differences in x_HI on random particles near the sightlines."""
+    w = _workload(nlos=8, n_side=32, nbins=512, seed=5)
+    plan = plan_for(w)
+    F = _target(w, plan)
+    xh = w.neutral_fraction
+    nfac = (w.dens["H1"] / xh).astype(np.float32)
+    nfac_t = dev(nfac)
+    x = dev(xh).requires_grad_(True)
+    dvel, dtemp = dev(w.vel), dev(w.temp)
+    tau = tau_autograd(plan, w.nbins, w.kernel, w.velfac, dvel, dtemp, [HI_1215], [x * nfac_t])[0]
+    ((torch.exp(-tau) - F) ** 2).sum().backward()
+    gx = x.grad.double().cpu().numpy()
+
+    def loss_of(xv):
+        d = dev(xv) * nfac_t
+        t = SP.tau(plan, w.nbins, w.kernel, w.velfac, dvel, dtemp, [HI_1215], [d])[0]
+        return ((torch.exp(-t) - F) ** 2).sum().item()
+
+    near = plan.near_lines().cpu().numpy()
+    sel = np.random.default_rng(11).choice(near, size=min(32, near.size), replace=False)
+    fd = []
+    for p in sel:
+        xv = xh.copy()
+        x0 = float(xv[p])
+        h = 1e-4 * x0
+        xv[p] = x0 + h
+        lp = loss_of(xv)
+        xv[p] = x0 - h
+        lm = loss_of(xv)
+        hp = float(np.float32(x0 + h)) - x0
+        hm = x0 - float(np.float32(x0 - h))
+        fd.append((lp - lm) / (hp + hm))
+    fd = np.asarray(fd)
+    a = gx[sel]
+    scale = np.max(np.abs(fd))
+    assert scale > 0
+    bad = np.abs(a - fd) > 2e-2 * np.abs(fd) + 2e-3 * scale
+    assert bad.sum() <= 1, "x_HI: %d of %d disagree: %s vs %s" % (bad.sum(), fd.size, a[bad][:4], fd[bad][:4])

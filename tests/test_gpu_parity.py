@@ -82,7 +82,7 @@ def test_crafted_cases_vs_reference_fixture():
                                        c["hh"], c["axis"], c["cofm"])
         assert_tau_close(got, z[key], key)
     for case, c in cases.items():
-        for kern in (0, 1):
+        for kern in (0, 1, 3):
             got = SP._Particle_Interpolate(0, 257, kern, float(c["box"]), velfac, 0.25, HI_1215.lambda_cm,
                                            HI_1215.gamma, HI_1215.fosc, HI_1215.amumass, 1e-7, c["pos"],
                                            np.zeros(1, np.float32), c["dens"], np.zeros(1, np.float32), c["hh"],
@@ -218,3 +218,32 @@ def test_stats_counters_consistent():
         assert st["far_pixels"] < st["evaluated_pixels"]
         assert st["kernel_sums"] >= st["evaluated_pixels"] - st["far_pixels"]
         assert st["voigt_samples"] >= 7 * (st["pixels"] - st["far_pixels"])
+
+
+def test_weighted_fields_match_oracle(oracle):
+    """Column-density-weighted velocity and temperature from one binning and
+    one batched colden launch equal the quotient of the oracle's column
+    densities of dens*w and dens (spectra.py:945-1010)."""
+    from fake_spectra_amd import products
+    w = synth.config(0, nlos=8)
+    plan = plan_for(w)
+    dens = w.dens["H1"]
+    r = products.velocity_temperature(plan, w.nbins, w.kernel, w.velfac, dev(dens), dev(w.vel), dev(w.temp),
+                                      w.atime)
+
+    def col(d):
+        a = list(w.interp_args(None, False))
+        a[13] = np.ascontiguousarray(d, dtype=np.float32)
+        return oracle.interpolate(*a)
+
+    base = col(dens)
+    assert_colden_close(r["colden"].cpu().numpy(), base, "colden")
+    den = np.where(base == 0, 1.0, base)
+    v = (w.vel * np.float32(np.sqrt(w.atime))).astype(np.float32)
+    checks = [("temp", r["temp"], dens * w.temp)] + \
+             [("vel%d" % a, r["velocity"][..., a], dens * v[:, a]) for a in range(3)]
+    for name, got, weighted in checks:
+        ref = col(weighted) / den
+        g = got.cpu().numpy()
+        scale = np.max(np.abs(ref))
+        np.testing.assert_allclose(g, ref, rtol=3e-6, atol=3e-6 * scale, err_msg=name)

@@ -33,7 +33,7 @@ def test_crafted_cases(oracle):
                                  c["axis"], c["cofm"])
         np.testing.assert_allclose(tau, z[key], rtol=1e-11, atol=1e-300, err_msg=key)
     for case, c in cases.items():
-        for kern in (0, 1):
+        for kern in (0, 1, 3):
             col = oracle.interpolate(0, 257, kern, float(c["box"]), velfac, 0.25, HI_1215.lambda_cm, HI_1215.gamma,
                                      HI_1215.fosc, HI_1215.amumass, 1e-7, c["pos"], np.zeros(1, np.float32),
                                      c["dens"], np.zeros(1, np.float32), c["hh"], c["axis"], c["cofm"])
