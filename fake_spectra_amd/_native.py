@@ -25,7 +25,8 @@ EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan
            "fsamd_plan_npairs", "fsamd_plan_pairs", "fsamd_plan_near_lines", "fsamd_tau",
            "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats", "fsamd_last_ms",
            "fsamd_voigt_peak", "fsamd_set_group_size", "fsamd_plan_keep_windows", "fsamd_tau_adjoint",
-           "fsamd_tau_autotune", "fsamd_plan_overflow")
+           "fsamd_tau_autotune", "fsamd_plan_overflow", "fsamd_mean_flux_scale", "fsamd_res_corr",
+           "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms")
 
 
 class FsamdLine(C.Structure):
@@ -72,6 +73,8 @@ def _declare(L):
     L.fsamd_last_stats.restype = i
     L.fsamd_last_ms.argtypes = []
     L.fsamd_last_ms.restype = d
+    L.fsamd_last_kernel_ms.argtypes = []
+    L.fsamd_last_kernel_ms.restype = d
     L.fsamd_voigt_peak.argtypes = [d, d, C.POINTER(d), vp]
     L.fsamd_voigt_peak.restype = i
     L.fsamd_set_group_size.argtypes = [i]
@@ -85,6 +88,14 @@ def _declare(L):
     L.fsamd_tau_autotune.restype = i
     L.fsamd_plan_overflow.argtypes = [vp, C.POINTER(i), vp]
     L.fsamd_plan_overflow.restype = i
+    L.fsamd_mean_flux_scale.argtypes = [vp, ll, d, d, d, C.POINTER(d), vp]
+    L.fsamd_mean_flux_scale.restype = i
+    L.fsamd_res_corr.argtypes = [vp, vp, i, i, d, vp]
+    L.fsamd_res_corr.restype = i
+    L.fsamd_flux_power.argtypes = [vp, i, i, d, d, d, vp, vp]
+    L.fsamd_flux_power.restype = i
+    L.fsamd_flux_hist.argtypes = [vp, ll, d, i, vp, vp]
+    L.fsamd_flux_hist.restype = i
 
 
 def lib():

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libfsamd.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-SOURCES = ["binning.hip", "deposit.hip", "capi.hip"]
+SOURCES = ["binning.hip", "deposit.hip", "capi.hip", "flux.hip"]
 ARCH = os.environ.get("FSAMD_ARCH", "gfx950")
 
 # -ffp-contract=off: the membership tests, the velocity-bin mapping and the

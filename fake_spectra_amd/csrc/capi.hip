@@ -14,7 +14,8 @@ static thread_local std::string g_error;
 static thread_local long long g_stats[kNStats] = {};
 static thread_local int g_stats_on = 0;    // device counters (and timing)
 static thread_local int g_timing_on = 0;   // hipEvents around the launches
-static thread_local double g_last_ms = 0;
+static thread_local double g_last_ms = 0;          // whole call
+static thread_local double g_last_kernel_ms = 0;   // deposit kernel launches only
 static int g_group_size = 16;   // pixels per lane group of the tau kernel (16, 32 or 64)
 
 void set_error(const std::string &msg) { g_error = msg; }
@@ -205,6 +206,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         for (auto &e : P->ev)
             if (!e) FSAMD_HIP(hipEventCreate(&e));
         FSAMD_HIP(hipEventRecord(P->ev[0], s));
+        g_last_kernel_ms = 0;
     }
     const int gsize = force_g ? force_g
                       : (P->tuned_g > 0 && P->tuned_key == tune_key(nbins, nlines, kernel)) ? P->tuned_g : g_group_size;
@@ -239,6 +241,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         A.nbins = nbins;
         A.kernel = kernel;
         A.velfac = velfac;
+        A.inv_velfac = 1.0 / velfac;
         A.vbox = P->box * velfac;     // absorption.cpp:157
         A.box = P->box;
         A.tautail = tautail;
@@ -272,11 +275,13 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
             if (items.empty()) break;
             // captured graphs keep the overflow flag sticky across replays
             if (tau && (rc = launch_fill_i32(A.fix_count, 0, capturing ? 1 : 2, s))) return rc;
+            if (g_timing_on) FSAMD_HIP(hipEventRecord(P->ev[2], s));
             if (!plain) {
                 A.nitems = (int)items.size();
                 if ((rc = launch_fill_i32(A.queue, 0, 1, s))) return rc;
                 const int per_cu = deposit_blocks_per_cu(tau, lds, gsize, nbins);
                 const int nblocks = (int)std::min<long long>((long long)per_cu * dev.ncu, A.nitems);
+                if (g_timing_on) FSAMD_HIP(hipEventRecord(P->ev[2], s));
                 if ((rc = launch_deposit(tau, lds, gsize, A, set->buf.as<WorkItem>(), 0, nblocks, 1, s))) return rc;
             } else {
                 // one launch per (sightline, line), in the plan's item order
@@ -290,6 +295,13 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
                     i = j;
                 }
             }
+            if (g_timing_on) {
+                FSAMD_HIP(hipEventRecord(P->ev[3], s));
+                FSAMD_HIP(hipEventSynchronize(P->ev[3]));
+                float kms = 0;
+                FSAMD_HIP(hipEventElapsedTime(&kms, P->ev[2], P->ev[3]));
+                g_last_kernel_ms += kms;
+            }
             if (!tau) break;
             if ((rc = launch_tau_fixup(A, set->buf.as<WorkItem>(), dev.ncu * 2, s))) return rc;
             if (capturing) break;   // overflow is checked after replay (fsamd_plan_overflow)
@@ -301,6 +313,11 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
             if (attempt > 2) return fail(FSAMD_ERR_HIP, "fix-up list overflow persists");
             if ((rc = grow_fixups(P, 2ll * fc[0] + 65536, s))) return rc;
             if (g_stats_on && (rc = launch_fill_f64((double *)P->stats.ptr, 0.0, kNStats, s))) return rc;
+            if (g_timing_on) {
+                float kms = 0;
+                FSAMD_HIP(hipEventElapsedTime(&kms, P->ev[2], P->ev[3]));
+                g_last_kernel_ms -= kms;    // the rerun's launch replaces this one
+            }
         }
     }
     if (g_timing_on) {
@@ -518,6 +535,7 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         A.nbins = nbins;
         A.kernel = kernel;
         A.velfac = velfac;
+        A.inv_velfac = 1.0 / velfac;
         A.vbox = P->box * velfac;
         A.box = P->box;
         A.pos = pos;
@@ -562,6 +580,7 @@ int fsamd_last_stats(long long *out, int n)
 }
 
 double fsamd_last_ms(void) { return g_last_ms; }
+double fsamd_last_kernel_ms(void) { return g_last_kernel_ms; }
 
 int fsamd_set_group_size(int g)
 {

@@ -131,8 +131,14 @@ __device__ inline double pair_dr2(const LosInfo &L, const float *pp, double box)
     return d2;
 }
 
+// z mod nbins in [0, nbins) (absorption.cpp:257-259); walks stay within half a
+// box of a pixel inside the box, so one add or subtract covers all but
+// particles with extreme peculiar velocities.
 __device__ inline int wrap_pixel(int z, int nbins)
 {
+    if (z >= 0 && z < nbins) return z;
+    if (z < 0 && z >= -nbins) return z + nbins;
+    if (z >= nbins && z < 2 * nbins) return z - nbins;
     int j = z % nbins;
     return j < 0 ? j + nbins : j;
 }
@@ -167,6 +173,7 @@ struct PairTau {
     double dv;       // sub-sample spacing bintov / (np - 1)
     double inv_nm1;  // 1 / (np - 1)
     float inv_nm1f;
+    int nm1, rounds, pstep;   // near-step shape of the main kernel (step_shape)
 };
 
 __device__ inline void wave_sync()
@@ -324,6 +331,37 @@ __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
     }
 }
 
+// Sub-sample grid of one walk step in the main kernel.  A step evaluates one
+// 7-sample kernel sum per lane and round, on the sub-sample grid of
+// tau_kern_outer (singleabs.h:104-126) laid out across the lanes: adjacent
+// pixels share their boundary sub-sample, so a step of p pixels with np
+// sub-samples each needs p (np - 1) + 1 sums; midpoint pixels (np = 1) need
+// one each.  Pairs whose grid exceeds kRoundsMax rounds take the FP64 path.
+constexpr int kRoundsMax = 4;
+
+struct StepShape {
+    int nm1;      // np - 1 (0: midpoint)
+    int rounds;   // lane rounds per step
+    int pstep;    // pixels per step
+};
+
+__device__ inline StepShape step_shape(int np, int kg)
+{
+    StepShape sh;
+    sh.nm1 = np - 1;
+    if (sh.nm1 == 0) {
+        sh.rounds = 1;
+        sh.pstep = kg;
+    } else if (sh.nm1 < kg) {
+        sh.rounds = 1;
+        sh.pstep = (kg - 1) / sh.nm1;
+    } else {
+        sh.rounds = min(kRoundsMax, (2 * sh.nm1 + kg) / kg);
+        sh.pstep = (kg * sh.rounds - 1) / sh.nm1;
+    }
+    return sh;
+}
+
 // Set up one pair in the wave's LDS slot; returns false if the particle does
 // not reach the sightline.  Every lane of the wave calls it.
 template <int kG>
@@ -345,20 +383,22 @@ __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const
     val1 *= A.velfac;
     const double vdr2 = val1;
     const double vsmooth = A.velfac * (double)smooth;
-    const double aa = L.voigt_fac / b;
-    const double vhigh = vsmooth * vsmooth > vdr2 ? sqrt(vsmooth * vsmooth - vdr2) : 0;   // singleabs.h:83
     const double inv_b = 1.0 / b;
+    // a = voigt_fac/b, amp = sigma_a/sqrt(pi) (c/1e5)/b (absorption.cpp:238,244),
+    // by one reciprocal: within 2 ulp of the reference's quotients
+    const double aa = L.voigt_fac * inv_b;
+    const double vhigh = vsmooth * vsmooth > vdr2 ? sqrt(vsmooth * vsmooth - vdr2) : 0;   // singleabs.h:83
     const double bintov = A.vbox / A.nbins;
-    const double amp = L.sigma_a / sqrt(kPi) * (kLight / 1e5 / b);
+    const double amp = L.sigma_a * (kLight / 1e5 / kSqrtPi) * inv_b;
     const double dvk = 2. * vhigh / kNGrid;
+    const double xs = bintov * inv_b;     // bintov / (b/2) / 2 (singleabs.h:114)
     int np = 1;
-    if (!(bintov < b / 2.)) np = 2 * ceil(bintov / (b / 2.) / 2) + 1.;
+    if (!(bintov < b / 2.)) np = 2 * ceil(xs) + 1.;
     // The reference sizes the sub-sample grid per pixel from vhigh - vlow,
     // which equals bintov up to the rounding of vlow + bintov (~1e-13
     // relative); the walk uses one grid per pair, so a pair whose grid could
     // differ between pixels (bintov/b at a switch point to 1e-12) goes to the
     // FP64 path, which follows the reference pixel by pixel.
-    const double xs = bintov / (b / 2.) / 2;
     const bool grid_edge = fabs(bintov - b / 2.) <= 1e-12 * bintov || (np > 1 && fabs(xs - rint(xs)) <= 1e-12 * xs);
     // pixels wider than max_sums sub-samples do not fit one walk step
     const int fp64 = !(aa <= (double)kA32Max) || grid_edge || np - 1 > max_sums - 1;
@@ -383,17 +423,21 @@ __device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const
         P.zmax = (int)floor(vel / bintov);
         P.fp64 = fp64;
         P.va = make_voigt_a(fp64 ? 0.0 : aa);
-        P.scale = amp * (double)dens * dvk / A.velfac;
-        P.scale_nd = amp * dvk / A.velfac;
+        P.scale = amp * (double)dens * dvk * A.inv_velfac;
+        P.scale_nd = amp * dvk * A.inv_velfac;
         P.temp = (double)temp;
         P.part = p;
         P.np = np;
         P.ufar = ufar;
         P.ufar2 = (float)(ufar * ufar);
         P.wing_ready = 0;
-        P.dv = np > 1 ? bintov / (np - 1) : 0.0;
         P.inv_nm1 = np > 1 ? 1.0 / (np - 1) : 1.0;
+        P.dv = np > 1 ? bintov * P.inv_nm1 : 0.0;
         P.inv_nm1f = (float)P.inv_nm1;
+        const StepShape sh = step_shape(np, kG);
+        P.nm1 = sh.nm1;
+        P.rounds = sh.rounds;
+        P.pstep = sh.pstep;
     }
     wave_sync();
     return true;
@@ -508,37 +552,6 @@ __device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double
     return false;
 }
 
-// Sub-sample grid of one walk step in the main kernel.  A step evaluates one
-// 7-sample kernel sum per lane and round, on the sub-sample grid of
-// tau_kern_outer (singleabs.h:104-126) laid out across the lanes: adjacent
-// pixels share their boundary sub-sample, so a step of p pixels with np
-// sub-samples each needs p (np - 1) + 1 sums; midpoint pixels (np = 1) need
-// one each.  Pairs whose grid exceeds kRoundsMax rounds take the FP64 path.
-constexpr int kRoundsMax = 4;
-
-struct StepShape {
-    int nm1;      // np - 1 (0: midpoint)
-    int rounds;   // lane rounds per step
-    int pstep;    // pixels per step
-};
-
-__device__ inline StepShape step_shape(int np, int kg)
-{
-    StepShape sh;
-    sh.nm1 = np - 1;
-    if (sh.nm1 == 0) {
-        sh.rounds = 1;
-        sh.pstep = kg;
-    } else if (sh.nm1 < kg) {
-        sh.rounds = 1;
-        sh.pstep = (kg - 1) / sh.nm1;
-    } else {
-        sh.rounds = min(kRoundsMax, (2 * sh.nm1 + kg) / kg);
-        sh.pstep = (kg * sh.rounds - 1) / sh.nm1;
-    }
-    return sh;
-}
-
 // One step of a walk in the main kernel (lane glane of a kG-lane group):
 // the next pixels z0 + dir*(c + p), p < pstep, of the walk.  Pixels whose
 // centre lies in the far wing take the moment expansion (a step is a far
@@ -556,7 +569,10 @@ __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *
     const double tt = A.tautail;
     const double U0 = ((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
     const bool far = fabs(U0) >= P.ufar;
-    const StepShape sh = step_shape(far ? 1 : P.np, kG);
+    StepShape sh;
+    sh.nm1 = far ? 0 : P.nm1;
+    sh.rounds = far ? 1 : P.rounds;
+    sh.pstep = far ? kG : P.pstep;
     const int pcount = active ? min(sh.pstep, count - c) : 0;
     wave_sync();   // the previous step's reads of sbuf are done
     if (active && !far) {

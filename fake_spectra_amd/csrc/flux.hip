@@ -1,0 +1,275 @@
+// Flux statistics on device-resident optical depths: the mean-flux rescale
+// (py_module.cpp:235-262, fluxstatistics.py:26-39), spectrograph smoothing
+// (spec_utils.py:5-25), the 1D flux power (fluxstatistics.py:71-108) and the
+// flux PDF (fluxstatistics.py:41-49).  All FP64, HBM-bound streaming kernels.
+#include "fsamd_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace fsamd {
+
+constexpr int kFluxThreads = 256;
+
+template <typename T>
+__device__ inline T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Block-wide sum of kN values per thread; thread 0 returns the totals.
+template <int kN>
+__device__ inline void block_sum(double (&v)[kN])
+{
+    __shared__ double part[kN][kFluxThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < kN; ++i) {
+        const double s = wave_sum(v[i]);
+        if (lane == 0) part[i][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < kN; ++i) {
+            double s = 0;
+            for (int w = 0; w < kFluxThreads / 64; ++w) s += part[i][w];
+            v[i] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Per-block partial sums of exp(-s tau), tau exp(-s tau) and the count of
+// pixels with tau <= thresh (py_module.cpp:242-251).
+__global__ __launch_bounds__(kFluxThreads) void flux_moments_kernel(const double *tau, long long n, double scale,
+                                                                   double thresh, double *partial)
+{
+    double v[3] = {0, 0, 0};
+    const long long stride = (long long)gridDim.x * kFluxThreads;
+    for (long long i = blockIdx.x * (long long)kFluxThreads + threadIdx.x; i < n; i += stride) {
+        const double t = tau[i];
+        if (t > thresh) continue;
+        const double e = exp(-scale * t);
+        v[0] += e;
+        v[1] += e * t;
+        v[2] += 1.0;
+    }
+    block_sum<3>(v);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 3; ++i) partial[3 * blockIdx.x + i] = v[i];
+}
+
+// Gaussian smoothing along rows with periodic wrap, weights w[0..2r]
+// (the correlation of scipy.ndimage.gaussian_filter1d, mode 'wrap').
+__global__ __launch_bounds__(kFluxThreads) void res_corr_kernel(const double *in, double *out, int nrow, int ncol,
+                                                               const double *w, int r)
+{
+    for (int row = blockIdx.x; row < nrow; row += gridDim.x) {
+        const double *src = in + (size_t)row * ncol;
+        double *dst = out + (size_t)row * ncol;
+        for (int i = threadIdx.x; i < ncol; i += kFluxThreads) {
+            double acc = 0;
+            for (int k = -r; k <= r; ++k) {
+                int j = (i + k) % ncol;
+                if (j < 0) j += ncol;
+                acc += w[k + r] * src[j];
+            }
+            dst[i] = acc;
+        }
+    }
+}
+
+// Per-sightline |DFT|^2 of d = exp(-s tau)/mean - 1 at k = 0..npix/2, by a
+// direct sum over an LDS twiddle table: power[row][k] = |sum_j d_j
+// exp(-2 pi i j k / npix)|^2.  One block per sightline.
+__global__ __launch_bounds__(kFluxThreads) void flux_dft_kernel(const double *tau, int nlos, int npix, double scale,
+                                                               double inv_mean, double *power)
+{
+    extern __shared__ double sh[];
+    double *d = sh;                 // [npix]
+    double *cs = sh + npix;         // [npix] cos(2 pi m / npix)
+    double *sn = sh + 2 * npix;     // [npix] sin(2 pi m / npix)
+    for (int m = threadIdx.x; m < npix; m += kFluxThreads) {
+        double s, c;
+        sincospi(2.0 * m / npix, &s, &c);
+        cs[m] = c;
+        sn[m] = s;
+    }
+    const int nk = npix / 2 + 1;
+    for (int row = blockIdx.x; row < nlos; row += gridDim.x) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < npix; j += kFluxThreads)
+            d[j] = exp(-scale * tau[(size_t)row * npix + j]) * inv_mean - 1.0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < nk; k += kFluxThreads) {
+            double re = 0, im = 0;
+            unsigned int m = 0;        // j k mod npix
+            for (int j = 0; j < npix; ++j) {
+                re += d[j] * cs[m];
+                im -= d[j] * sn[m];
+                m += (unsigned int)k;
+                if (m >= (unsigned int)npix) m -= npix;
+            }
+            power[(size_t)row * nk + k] = re * re + im * im;
+        }
+    }
+}
+
+// sum over rows of power[row][k], in row order (deterministic)
+__global__ void column_sum_kernel(const double *power, int nrow, int nk, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    double s = 0;
+    for (int r = 0; r < nrow; ++r) s += power[(size_t)r * nk + k];
+    out[k] = s;
+}
+
+// Histogram of exp(-s tau) over nb equal bins of [0, 1], bin edges e_k =
+// k / nb as numpy.histogram places them (searchsorted on the edges, the last
+// bin closed); values outside [0, 1] are not counted.
+__global__ __launch_bounds__(kFluxThreads) void flux_hist_kernel(const double *tau, long long n, double scale, int nb,
+                                                                unsigned long long *counts)
+{
+    const long long stride = (long long)gridDim.x * kFluxThreads;
+    for (long long i = blockIdx.x * (long long)kFluxThreads + threadIdx.x; i < n; i += stride) {
+        const double f = exp(-scale * tau[i]);
+        if (!(f >= 0.0 && f <= 1.0)) continue;
+        int b = (int)(f * nb);
+        b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+        if (f < (double)b / (double)nb && b > 0) --b;
+        if (b < nb - 1 && f >= (double)(b + 1) / (double)nb) ++b;
+        atomicAdd(&counts[b], 1ull);
+    }
+}
+
+}  // namespace fsamd
+
+using namespace fsamd;
+
+extern "C" {
+
+int fsamd_mean_flux_scale(const double *tau, long long n, double mean_flux_desired, double tol, double thresh,
+                          double *scale_out, void *stream)
+{
+    if (!scale_out || n < 0 || (n > 0 && !tau)) return fail(FSAMD_ERR_ARG, "NULL argument or negative size");
+    if (n == 0) {
+        *scale_out = 0.0;    // fluxstatistics.py:38-39
+        return FSAMD_OK;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int nblocks = (int)std::min<long long>(1024, (n + kFluxThreads - 1) / kFluxThreads);
+    double *partial = nullptr;
+    FSAMD_HIP(hipMalloc(&partial, sizeof(double) * 3 * nblocks));
+    std::vector<double> host(3 * nblocks);
+    double scale, newscale = 1;
+    int rc = FSAMD_OK;
+    for (int it = 0; it < 1000; ++it) {
+        scale = newscale;
+        flux_moments_kernel<<<nblocks, kFluxThreads, 0, s>>>(tau, n, scale, thresh, partial);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(host.data(), partial, sizeof(double) * 3 * nblocks,
+                                                hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            rc = fail(FSAMD_ERR_HIP, std::string("flux moments: ") + hipGetErrorString(e));
+            break;
+        }
+        double mf = 0, tmf = 0, used = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            mf += host[3 * b];
+            tmf += host[3 * b + 1];
+            used += host[3 * b + 2];
+        }
+        // Newton-Raphson on <exp(-s tau)> = mean_flux_desired (py_module.cpp:252-257)
+        newscale = scale + (mf - mean_flux_desired * used) / tmf;
+        if (newscale <= 0) newscale = 1e-10;
+        if (!(fabs(newscale - scale) > tol * newscale)) break;
+        if (it == 999) rc = fail(FSAMD_ERR_ARG, "mean flux rescale did not converge");
+    }
+    (void)hipFree(partial);
+    if (rc) return rc;
+    *scale_out = newscale;
+    return FSAMD_OK;
+}
+
+int fsamd_res_corr(const double *in, double *out, int nrow, int ncol, double sigma, void *stream)
+{
+    if (nrow < 0 || ncol < 1 || !(sigma > 0) || (nrow > 0 && (!in || !out)))
+        return fail(FSAMD_ERR_ARG, "bad arguments to fsamd_res_corr");
+    if (nrow == 0) return FSAMD_OK;
+    // scipy.ndimage.gaussian_filter1d: truncate = 4, radius = int(4 sigma + 0.5)
+    const int r = (int)(4.0 * sigma + 0.5);
+    if (r > (1 << 20)) return fail(FSAMD_ERR_UNSUPPORTED, "smoothing radius too large");
+    std::vector<double> w(2 * r + 1);
+    double tot = 0;
+    for (int k = -r; k <= r; ++k) tot += (w[k + r] = std::exp(-0.5 * (double)k * k / (sigma * sigma)));
+    for (double &x : w) x /= tot;
+    hipStream_t s = (hipStream_t)stream;
+    double *dw = nullptr;
+    FSAMD_HIP(hipMalloc(&dw, sizeof(double) * w.size()));
+    hipError_t e = hipMemcpyAsync(dw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        res_corr_kernel<<<std::min(nrow, 65536), kFluxThreads, 0, s>>>(in, out, nrow, ncol, dw, r);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dw);
+    if (e != hipSuccess) return fail(FSAMD_ERR_HIP, std::string("res_corr: ") + hipGetErrorString(e));
+    return FSAMD_OK;
+}
+
+int fsamd_flux_power(const double *tau, int nlos, int npix, double scale, double mean_flux, double vmax,
+                     double *power, void *stream)
+{
+    if (nlos < 1 || npix < 1 || !tau || !power || !(mean_flux != 0))
+        return fail(FSAMD_ERR_ARG, "bad arguments to fsamd_flux_power");
+    const size_t shm = sizeof(double) * 3 * (size_t)npix;
+    if (shm > 160 * 1024) return fail(FSAMD_ERR_UNSUPPORTED, "flux power: npix too large for the LDS tables");
+    hipStream_t s = (hipStream_t)stream;
+    const int nk = npix / 2 + 1;
+    double *per = nullptr;
+    FSAMD_HIP(hipMalloc(&per, sizeof(double) * (size_t)nlos * nk));
+    hipError_t e = hipFuncSetAttribute((const void *)flux_dft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm);
+    if (e == hipSuccess) {
+        flux_dft_kernel<<<std::min(nlos, 65536), kFluxThreads, shm, s>>>(tau, nlos, npix, scale, 1.0 / mean_flux,
+                                                                         per);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        column_sum_kernel<<<(nk + 255) / 256, 256, 0, s>>>(per, nlos, nk, power);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(per);
+    if (e != hipSuccess) return fail(FSAMD_ERR_HIP, std::string("flux power: ") + hipGetErrorString(e));
+    // mean over sightlines of vmax |F_k|^2 / npix^2 (fluxstatistics.py:49-56, 95-102)
+    std::vector<double> h(nk);
+    FSAMD_HIP(hipMemcpy(h.data(), power, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    const double norm = vmax / ((double)npix * (double)npix) / nlos;
+    for (double &x : h) x *= norm;
+    FSAMD_HIP(hipMemcpy(power, h.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
+    return FSAMD_OK;
+}
+
+int fsamd_flux_hist(const double *tau, long long n, double scale, int nbins, unsigned long long *counts,
+                    void *stream)
+{
+    if (n < 0 || nbins < 1 || !counts || (n > 0 && !tau)) return fail(FSAMD_ERR_ARG, "bad arguments to fsamd_flux_hist");
+    hipStream_t s = (hipStream_t)stream;
+    FSAMD_HIP(hipMemsetAsync(counts, 0, sizeof(unsigned long long) * nbins, s));
+    if (n > 0) {
+        const int nblocks = (int)std::min<long long>(4096, (n + kFluxThreads - 1) / kFluxThreads);
+        flux_hist_kernel<<<nblocks, kFluxThreads, 0, s>>>(tau, n, scale, nbins, counts);
+        FSAMD_HIP(hipGetLastError());
+    }
+    FSAMD_HIP(hipStreamSynchronize(s));
+    return FSAMD_OK;
+}
+
+}  // extern "C"

@@ -84,6 +84,7 @@ struct DepositArgs {
     int nbins;
     int kernel;
     double velfac;
+    double inv_velfac;  // 1 / velfac
     double vbox;       // box * velfac
     double box;
     double tautail;

@@ -92,7 +92,7 @@ struct fsamd_plan {
     // lane-group width chosen by fsamd_tau_autotune for (nbins, nlines, kernel)
     int tuned_g = 0;
     long long tuned_key = -1;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // call, deposit-kernel brackets
     // Once a call on this plan has been captured into a graph, buffers that
     // graph's kernels address are retired (kept until the plan is destroyed)
     // instead of freed when a later call needs them larger.

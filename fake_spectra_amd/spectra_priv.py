@@ -264,7 +264,7 @@ def last_stats():
     N.check(N.lib().fsamd_last_stats(buf, 8))
     return {"pairs": buf[0], "pixels": buf[1], "fp64_pixels": buf[2], "fp64_pairs": buf[3],
             "voigt_samples": buf[4], "evaluated_pixels": buf[5], "kernel_sums": buf[6], "far_pixels": buf[7],
-            "gpu_ms": float(N.lib().fsamd_last_ms())}
+            "gpu_ms": float(N.lib().fsamd_last_ms()), "kernel_ms": float(N.lib().fsamd_last_kernel_ms())}
 
 
 def set_stats(enable: bool, timing_only: bool = False):
@@ -386,3 +386,27 @@ def _near_lines(*args):
     finally:
         plan.close()
     return res
+
+
+def _rescale_mean_flux(*args):
+    """Scale factor for tau reaching a desired mean flux (Py_mean_flux,
+    py_module.cpp:263-282): (tau float64 array, mean_flux_desired, nbins,
+    tol, thresh) -> float; the first nbins entries of tau are used."""
+    if len(args) != 5 or not isinstance(args[0], np.ndarray):
+        raise AttributeError("Incorrect arguments: use tau (array), mean_flux_desired (double), nbins (int), "
+                             "tol (double), thresh (double)\n")
+    tau, mfd, nbins, tol, thresh = args
+    try:
+        mfd, nbins, tol, thresh = float(mfd), int(nbins), float(tol), float(thresh)
+    except (TypeError, ValueError):
+        raise AttributeError("Incorrect arguments: use tau (array), mean_flux_desired (double), nbins (int), "
+                             "tol (double), thresh (double)\n")
+    if tau.dtype != np.dtype(np.float64):
+        raise TypeError("Optical depth must have 64-bit float type\n")
+    flat = np.ascontiguousarray(tau).ravel()
+    if nbins < 0 or nbins > flat.size:
+        raise ValueError("nbins must lie between 0 and the size of tau")
+    t = torch.from_numpy(flat[:nbins].copy()).to(device())
+    out = C.c_double()
+    N.check(N.lib().fsamd_mean_flux_scale(_ptr(t), nbins, mfd, tol, thresh, C.byref(out), _stream()))
+    return out.value

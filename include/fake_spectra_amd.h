@@ -129,6 +129,9 @@ int fsamd_set_stats(int flags);
 #define FSAMD_NSTATS 8
 int fsamd_last_stats(long long *out, int n);
 double fsamd_last_ms(void);
+/* GPU time of the deposit kernel launches alone (the tau or colden kernel,
+ * without the fills and the FP64 fix-up pass) of the last timed call, ms. */
+double fsamd_last_kernel_ms(void);
 
 /* Autotuner (no counterpart in the reference): times the batched tau launch
  * for lane-group widths 16, 32 and 64 on every k-th sightline of the plan's
@@ -158,6 +161,29 @@ int fsamd_set_group_size(int g);
  * arguments u swept uniformly in [0, umax) with damping parameter a, no
  * memory traffic.  The denominator of the deposit's VALU fraction. */
 int fsamd_voigt_peak(double umax, double a, double *samples_per_s, void *stream);
+
+/* Flux statistics on device optical depths (FP64).
+ * fsamd_mean_flux_scale: the scale s with <exp(-s tau)> = mean_flux_desired
+ *   over the pixels with tau <= thresh, Newton-Raphson from s = 1 until
+ *   |s' - s| <= tol s' (_rescale_mean_flux, py_module.cpp:235-282); 0 for
+ *   n = 0 (fluxstatistics.py:38-39).  tau: device [n].
+ * fsamd_res_corr: Gaussian smoothing along each row with periodic wrap,
+ *   standard deviation sigma pixels, weights exp(-k^2/2sigma^2) normalised over
+ *   |k| <= int(4 sigma + 0.5) (spec_utils.py:5-25, which calls
+ *   scipy.ndimage.gaussian_filter1d(mode='wrap')).  in, out: device [nrow, ncol].
+ * fsamd_flux_power: mean over sightlines of vmax |rfft(exp(-scale tau)/mean_flux - 1)|^2 / npix^2
+ *   at k = 0..npix/2 (fluxstatistics.py:49-56, 71-108 before the window
+ *   correction).  tau: device [nlos, npix]; power: device [npix/2 + 1].
+ * fsamd_flux_hist: counts of exp(-scale tau) in nbins equal bins of [0, 1]
+ *   placed as numpy.histogram places them (fluxstatistics.py:41-49).
+ *   counts: device [nbins]. */
+int fsamd_mean_flux_scale(const double *tau, long long n, double mean_flux_desired, double tol, double thresh,
+                          double *scale, void *stream);
+int fsamd_res_corr(const double *in, double *out, int nrow, int ncol, double sigma, void *stream);
+int fsamd_flux_power(const double *tau, int nlos, int npix, double scale, double mean_flux, double vmax,
+                     double *power, void *stream);
+int fsamd_flux_hist(const double *tau, long long n, double scale, int nbins, unsigned long long *counts,
+                    void *stream);
 
 #ifdef __cplusplus
 }

@@ -211,6 +211,7 @@ def test_stats_counters_consistent():
             # FP64 LDS atomics add in launch-dependent order
             np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-300)
         assert st["gpu_ms"] > 0
+        assert 0 < st["kernel_ms"] <= st["gpu_ms"]
         if timing_only:
             continue
         assert st["pairs"] > 0 and st["pixels"] > 0
