@@ -204,10 +204,11 @@ def main():
 
     # pick the lane-group width on this workload (kept by the plan for later calls)
     tplan = SP.Plan(w.box, pos, hh, ax, cofm)
-    best_g, tune_ms = SP.autotune(tplan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail)
+    best_g, best_w, tune_ms = SP.autotune(tplan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail)
     tplan.close()
     N.set_group_size(best_g)
-    progress("autotune: group width %d %s" % (best_g, tune_ms))
+    N.set_block_waves(best_w)
+    progress("autotune: group width %d, %d wavefronts per sightline %s" % (best_g, best_w, tune_ms))
 
     state = {}
 
@@ -310,7 +311,8 @@ def main():
     del graph
     gplan.close()
     progress("graph replays done")
-    out["autotune"] = {"group_width": best_g, "sample_ms": {str(k): v for k, v in tune_ms.items()}}
+    out["autotune"] = {"group_width": best_g, "waves_per_sightline": best_w,
+                       "sample_ms": {"%dx%d" % k: v for k, v in tune_ms.items()}}
     if not args.no_plain:
         plan = SP.Plan(w.box, pos, hh, ax, cofm)
         # one untimed call builds the per-sightline work items

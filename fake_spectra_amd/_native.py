@@ -26,7 +26,7 @@ EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan
            "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats", "fsamd_last_ms",
            "fsamd_voigt_peak", "fsamd_set_group_size", "fsamd_plan_keep_windows", "fsamd_tau_adjoint",
            "fsamd_tau_autotune", "fsamd_plan_overflow", "fsamd_mean_flux_scale", "fsamd_res_corr",
-           "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms")
+           "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms", "fsamd_set_block_waves")
 
 
 class FsamdLine(C.Structure):
@@ -79,12 +79,14 @@ def _declare(L):
     L.fsamd_voigt_peak.restype = i
     L.fsamd_set_group_size.argtypes = [i]
     L.fsamd_set_group_size.restype = i
+    L.fsamd_set_block_waves.argtypes = [i]
+    L.fsamd_set_block_waves.restype = i
     L.fsamd_plan_keep_windows.argtypes = [vp, i]
     L.fsamd_plan_keep_windows.restype = i
     L.fsamd_tau_adjoint.argtypes = [vp, i, i, d, vp, vp, vp, vp, C.POINTER(FsamdLineGrad), i, vp, vp, vp]
     L.fsamd_tau_adjoint.restype = i
     L.fsamd_tau_autotune.argtypes = [vp, i, i, d, d, vp, vp, vp, vp, C.POINTER(FsamdLine), i, vp, C.POINTER(i),
-                                     C.POINTER(C.c_float)]
+                                     C.POINTER(i), C.POINTER(C.c_float)]
     L.fsamd_tau_autotune.restype = i
     L.fsamd_plan_overflow.argtypes = [vp, C.POINTER(i), vp]
     L.fsamd_plan_overflow.restype = i
@@ -114,6 +116,11 @@ def lib():
 def set_group_size(g: int):
     """Pixels per lane group of the tau kernel: 16, 32 or 64."""
     check(lib().fsamd_set_group_size(int(g)))
+
+
+def set_block_waves(w: int):
+    """Wavefronts per (sightline, line) work item of the tau kernel: 2, 4 or 8."""
+    check(lib().fsamd_set_block_waves(int(w)))
 
 
 def check(rc: int):

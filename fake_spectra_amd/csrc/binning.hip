@@ -67,6 +67,42 @@ int launch_fill_f64(double *p, double value, long long n, hipStream_t s)
     return FSAMD_OK;
 }
 
+__global__ void pack_records_kernel(const float *pos, const float *vel, const float *temp, const float *h,
+                                    const LineDesc *lines, int nlines, long long n, PartRec *out)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) {
+        float4 a, b, d0, d1;
+        a.x = pos[3 * i];
+        a.y = pos[3 * i + 1];
+        a.z = pos[3 * i + 2];
+        a.w = h[i];
+        b.x = vel[3 * i];
+        b.y = vel[3 * i + 1];
+        b.z = vel[3 * i + 2];
+        b.w = temp[i];
+        float d[kMaxLines];
+#pragma unroll
+        for (int k = 0; k < kMaxLines; ++k) d[k] = k < nlines ? lines[k].dens[i] : 0.0f;
+        d0 = make_float4(d[0], d[1], d[2], d[3]);
+        d1 = make_float4(d[4], d[5], d[6], d[7]);
+        float4 *o = (float4 *)&out[i];
+        o[0] = a;
+        o[1] = b;
+        o[2] = d0;
+        o[3] = d1;
+    }
+}
+
+int launch_pack_records(const float *pos, const float *vel, const float *temp, const float *h,
+                        const LineDesc *lines, int nlines, long long n, PartRec *out, hipStream_t s)
+{
+    if (n <= 0) return FSAMD_OK;
+    pack_records_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(pos, vel, temp, h, lines, nlines, n, out);
+    FSAMD_HIP(hipGetLastError());
+    return FSAMD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan (tile reduce -> scan of tile sums -> apply).
 

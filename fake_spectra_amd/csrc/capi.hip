@@ -17,6 +17,7 @@ static thread_local int g_timing_on = 0;   // hipEvents around the launches
 static thread_local double g_last_ms = 0;          // whole call
 static thread_local double g_last_kernel_ms = 0;   // deposit kernel launches only
 static int g_group_size = 16;   // pixels per lane group of the tau kernel (16, 32 or 64)
+static int g_block_waves = 4;   // wavefronts per block (per sightline work item): 2, 4 or 8
 
 void set_error(const std::string &msg) { g_error = msg; }
 
@@ -167,7 +168,7 @@ static int line_table(fsamd_plan *P, const std::vector<LineDesc> &ld, bool captu
 static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double velfac, double tautail,
                        const float *pos, const float *vel, const float *temp, const float *h,
                        const fsamd_line *lines, int nlines, int mode, hipStream_t s, long long sample_stride = 0,
-                       int force_g = 0)
+                       int force_g = 0, int force_waves = 0)
 {
     if (!P) return fail(FSAMD_ERR_ARG, "plan is NULL");
     if (nbins < 1) return fail(FSAMD_ERR_ARG, "nbins must be positive");
@@ -208,8 +209,9 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         FSAMD_HIP(hipEventRecord(P->ev[0], s));
         g_last_kernel_ms = 0;
     }
-    const int gsize = force_g ? force_g
-                      : (P->tuned_g > 0 && P->tuned_key == tune_key(nbins, nlines, kernel)) ? P->tuned_g : g_group_size;
+    const bool tuned = P->tuned_g > 0 && P->tuned_key == tune_key(nbins, nlines, kernel);
+    const int gsize = force_g ? force_g : tuned ? P->tuned_g : g_group_size;
+    const int waves = force_waves ? force_waves : tuned ? P->tuned_waves : g_block_waves;
     const long long chunk_key = plain ? kPlainChunk : (sample_stride > 1 ? -sample_stride : (1ll << 62));
     const bool rec = tau && P->keep_windows;
     if (rec) {
@@ -231,6 +233,14 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
             return fail(FSAMD_ERR_ARG, "run one uncaptured fsamd_tau call before capturing");
         if (P->fix_cap < want && (rc = grow_fixups(P, want, s))) return rc;
         if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(int) * 2))) return rc;
+    }
+    if (tau) {
+        const size_t rbytes = sizeof(PartRec) * (size_t)std::max<long long>(P->npart, 1);
+        if (P->rec.bytes < rbytes) {
+            if (capturing) return fail(FSAMD_ERR_ARG, "run one uncaptured fsamd_tau call before capturing");
+            FSAMD_HIP(hipStreamSynchronize(s));
+            if ((rc = regrow(P, P->rec, rbytes))) return rc;
+        }
     }
     if (capturing) P->captured = true;
     for (int g = 0; g < ngroups; ++g) {
@@ -255,6 +265,9 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         A.pair_part = P->pair_part.as<int>();
         A.nlines = K;
         A.lines = table + k0;
+        A.rec = tau ? P->rec.as<PartRec>() : nullptr;
+        if (tau && (rc = launch_pack_records(pos, vel, temp, h, A.lines, K, P->npart, P->rec.as<PartRec>(), s)))
+            return rc;
         A.queue = P->queue.as<int>() + g;
         A.stats = g_stats_on ? P->stats.as<long long>() : nullptr;
         A.npairs = P->npairs;
@@ -279,10 +292,11 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
             if (!plain) {
                 A.nitems = (int)items.size();
                 if ((rc = launch_fill_i32(A.queue, 0, 1, s))) return rc;
-                const int per_cu = deposit_blocks_per_cu(tau, lds, gsize, nbins);
+                const int per_cu = deposit_blocks_per_cu(tau, lds, gsize, effective_waves(lds, g_stats_on, waves), nbins);
                 const int nblocks = (int)std::min<long long>((long long)per_cu * dev.ncu, A.nitems);
                 if (g_timing_on) FSAMD_HIP(hipEventRecord(P->ev[2], s));
-                if ((rc = launch_deposit(tau, lds, gsize, A, set->buf.as<WorkItem>(), 0, nblocks, 1, s))) return rc;
+                if ((rc = launch_deposit(tau, lds, gsize, waves, A, set->buf.as<WorkItem>(), 0, nblocks, 1, s)))
+                    return rc;
             } else {
                 // one launch per (sightline, line), in the plan's item order
                 size_t i = 0;
@@ -290,7 +304,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
                     size_t j = i;
                     while (j < items.size() && items[j].los == items[i].los && items[j].line == items[i].line) ++j;
                     A.nitems = (int)(j - i);
-                    if ((rc = launch_deposit(tau, lds, gsize, A, set->buf.as<WorkItem>(), (int)i, A.nitems, 0, s)))
+                    if ((rc = launch_deposit(tau, lds, gsize, waves, A, set->buf.as<WorkItem>(), (int)i, A.nitems, 0, s)))
                         return rc;
                     i = j;
                 }
@@ -440,9 +454,9 @@ int fsamd_plan_overflow(fsamd_plan *plan, int *flag, void *stream)
 
 int fsamd_tau_autotune(fsamd_plan *P, int nbins, int kernel, double velfac, double tautail, const float *pos,
                        const float *vel, const float *temp, const float *h, const fsamd_line *lines, int nlines,
-                       void *stream, int *best_g, float *ms3)
+                       void *stream, int *best_g, int *best_waves, float *ms5)
 {
-    if (!P || !best_g) return fail(FSAMD_ERR_ARG, "NULL argument");
+    if (!P || !best_g || !best_waves) return fail(FSAMD_ERR_ARG, "NULL argument");
     hipStream_t s = (hipStream_t)stream;
     // Large plans are timed on every stride-th sightline of the heaviest-first
     // order (the same mix of heavy and light work items), at least 16384
@@ -453,34 +467,39 @@ int fsamd_tau_autotune(fsamd_plan *P, int nbins, int kernel, double velfac, doub
     hipEvent_t e0, e1;
     FSAMD_HIP(hipEventCreate(&e0));
     FSAMD_HIP(hipEventCreate(&e1));
+    const int waves_alt[2] = {2, 8};
     float best = 1e30f;
-    int bg = 16;
+    int bg = 16, bw = 4;
     int rc = FSAMD_OK;
-    for (int rep = 0; rep < 2 && rc == FSAMD_OK; ++rep) {   // one untimed run makes the allocations
-        for (int i = 0; i < (rep ? 3 : 1); ++i) {
-            FSAMD_HIP(hipEventRecord(e0, s));
-            rc = run_deposit(true, P, nbins, kernel, velfac, tautail, pos, vel, temp, h, lines, nlines,
-                             FSAMD_MODE_BATCHED, s, stride, widths[i]);
-            if (rc) break;
-            FSAMD_HIP(hipEventRecord(e1, s));
-            FSAMD_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            FSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (rep == 1) {
-                if (ms3) ms3[i] = ms;
-                if (ms < best) {
-                    best = ms;
-                    bg = widths[i];
-                }
-            }
+    // one untimed run makes the allocations; then the three widths at 4
+    // wavefronts per block, then 2 and 8 wavefronts at the best width
+    for (int i = -1; i < 5 && rc == FSAMD_OK; ++i) {
+        const int g = i < 0 ? 16 : i < 3 ? widths[i] : bg;
+        const int w = i < 3 ? 4 : waves_alt[i - 3];
+        FSAMD_HIP(hipEventRecord(e0, s));
+        rc = run_deposit(true, P, nbins, kernel, velfac, tautail, pos, vel, temp, h, lines, nlines,
+                         FSAMD_MODE_BATCHED, s, stride, g, w);
+        if (rc) break;
+        FSAMD_HIP(hipEventRecord(e1, s));
+        FSAMD_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        FSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (i < 0) continue;
+        if (ms5) ms5[i] = ms;
+        if (ms < best) {
+            best = ms;
+            bg = g;
+            bw = w;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     P->tuned_g = bg;
+    P->tuned_waves = bw;
     P->tuned_key = tune_key(nbins, nlines, kernel);
     *best_g = bg;
+    *best_waves = bw;
     return FSAMD_OK;
 }
 
@@ -526,6 +545,11 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
     if ((rc = line_table(P, ld, false, s, &table))) return rc;
     const int ngroups = (nlines + kMaxLines - 1) / kMaxLines;
     if (P->queue.bytes < sizeof(int) * (size_t)ngroups && (rc = regrow(P, P->queue, sizeof(int) * ngroups))) return rc;
+    const size_t rbytes = sizeof(PartRec) * (size_t)std::max<long long>(P->npart, 1);
+    if (P->rec.bytes < rbytes) {
+        FSAMD_HIP(hipStreamSynchronize(s));
+        if ((rc = regrow(P, P->rec, rbytes))) return rc;
+    }
     const int gsize = g_group_size;
     for (int g = 0; g < ngroups; ++g) {
         const int k0 = g * kMaxLines;
@@ -548,6 +572,8 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         A.pair_part = P->pair_part.as<int>();
         A.nlines = K;
         A.lines = table + k0;
+        A.rec = P->rec.as<PartRec>();
+        if ((rc = launch_pack_records(pos, vel, temp, h, A.lines, K, P->npart, P->rec.as<PartRec>(), s))) return rc;
         A.queue = P->queue.as<int>() + g;
         A.npairs = P->npairs;
         A.win = P->win.as<int>() + (size_t)k0 * 2 * P->npairs;
@@ -586,6 +612,13 @@ int fsamd_set_group_size(int g)
 {
     if (g != 16 && g != 32 && g != 64) return fail(FSAMD_ERR_ARG, "group size must be 16, 32 or 64");
     g_group_size = g;
+    return FSAMD_OK;
+}
+
+int fsamd_set_block_waves(int w)
+{
+    if (w != 2 && w != 4 && w != 8) return fail(FSAMD_ERR_ARG, "wavefronts per block must be 2, 4 or 8");
+    g_block_waves = w;
     return FSAMD_OK;
 }
 

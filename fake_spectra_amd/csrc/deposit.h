@@ -17,9 +17,10 @@ struct WorkItem {
 };
 
 size_t deposit_lds_bytes(int nbins);
-int launch_deposit(bool tau, bool lds, int gsize, const DepositArgs &A, const WorkItem *items, int item_base,
+int launch_deposit(bool tau, bool lds, int gsize, int waves, const DepositArgs &A, const WorkItem *items, int item_base,
                    int nblocks, int use_queue, hipStream_t s);
-int deposit_blocks_per_cu(bool tau, bool lds, int gsize, int nbins);
+int deposit_blocks_per_cu(bool tau, bool lds, int gsize, int waves, int nbins);
+int effective_waves(bool lds, bool stats, int waves);
 int launch_tau_adjoint(int gsize, const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);
 int adjoint_blocks_per_cu(int gsize, int nbins);
 int launch_tau_fixup(const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);

@@ -365,16 +365,18 @@ __device__ inline StepShape step_shape(int np, int kg)
 // Set up one pair in the wave's LDS slot; returns false if the particle does
 // not reach the sightline.  Every lane of the wave calls it.
 template <int kG>
-__device__ inline bool setup_pair(const DepositArgs &A, const LineDesc &L, const LosInfo &los, int p,
+__device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc &L, const LosInfo &los, int p,
                                   PairTau &P, int lane, int max_sums = 1 << 30)
 {
-    const float *pp = &A.pos[3 * (size_t)p];
+    const float4 r0 = *(const float4 *)&A.rec[p].x;
+    const float4 r1 = *(const float4 *)&A.rec[p].vx;
+    const float pp[3] = {r0.x, r0.y, r0.z};
     const double dr2 = pair_dr2(los, pp, A.box);
-    const float smooth = A.h[p];
-    const float temp = A.temp[p];
-    const float dens = L.dens[p];
+    const float smooth = r0.w;
+    const float temp = r1.w;
+    const float dens = A.rec[p].dens[line];
     const float ppos = pp[los.ax - 1];
-    const float pvel = A.vel[3 * (size_t)p + los.ax - 1];
+    const float pvel = los.ax == 1 ? r1.x : (los.ax == 2 ? r1.y : r1.z);
     const double b = L.bfac * sqrt((double)temp);
     const float sm2 = smooth * smooth;
     if ((double)sm2 - dr2 <= 0) return false;            // absorption.cpp:232
@@ -552,43 +554,115 @@ __device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double
     return false;
 }
 
+// Far steps cover this many groups of kG pixels (the moment expansion is a
+// few FMAs per pixel, so a far walk advances several sub-steps while the
+// other groups of the wavefront evaluate their near pixels).
+constexpr int kFarRounds = 4;
+
+// End of one sub-step of a walk (lanes glane < pcount hold the pixels
+// cstep + glane of the walk, value t): deposit up to and including the first
+// pixel below tautail (absorption.cpp:260-263), or hand the walk to the
+// fix-up kernel when that decision falls in the guard band.  Returns true
+// when the walk is over, with `walked` set.
+template <int kG, bool kLds, bool kStats>
+__device__ inline bool settle(const DepositArgs &A, const PairTau &P, double *row, int lane, bool valid, double t,
+                              int z, double vlow, int z0, int dir, int cstep, int pcount, int count, LaneStats &st,
+                              long long k, int item, int &walked)
+{
+    const int glane = lane % kG;
+    const double tt = A.tautail;
+    const bool guard = valid && fabs(t - tt) <= kGuard * tt;
+    const unsigned long long mb = group_bits<kG>(__ballot(valid && !guard && t < tt), lane);
+    const unsigned long long mg = group_bits<kG>(__ballot(guard), lane);
+    const unsigned long long mall = mb | mg;
+    const int first = mall ? __builtin_ctzll(mall) : kG;
+    if (valid && (glane < first || (glane == first && ((mb >> glane) & 1ull)))) {
+        row_add<kLds>(&row[wrap_pixel(z, A.nbins)], t);
+        if (kStats) {
+            st.pixels++;
+            st.samples += pixel_samples(P, vlow, A.vbox / A.nbins);
+        }
+    }
+    if (first < kG && ((mg >> first) & 1ull)) {
+        // the FP32 value of the deciding pixel is too close to tautail
+        if (glane == 0) push_fixup(A, k, item, z0 + dir * (cstep + first), count - (cstep + first), dir);
+        walked = -1;
+        return true;
+    }
+    if (mb != 0) {
+        walked = cstep + first + 1;
+        return true;
+    }
+    if (cstep + pcount >= count) {
+        walked = count;
+        return true;
+    }
+    return false;
+}
+
 // One step of a walk in the main kernel (lane glane of a kG-lane group):
-// the next pixels z0 + dir*(c + p), p < pstep, of the walk.  Pixels whose
-// centre lies in the far wing take the moment expansion (a step is a far
-// step when its first pixel is: |U| only grows along a walk).  `sbuf` is the
-// group's LDS scratch of kG * kRoundsMax floats.  Returns true when the walk
-// is over (tail cut, end of the window, or the decision deferred to the
-// fix-up kernel); `adv` is the number of pixels the step covered.
+// the next pixels z0 + dir*(c + p) of the walk.  Pixels whose centre lies in
+// the far wing take the moment expansion: a step is a far step when its
+// first pixel is (|U| only grows along a walk) and then covers kFarRounds
+// sub-steps of kG pixels.  A near step covers P.pstep pixels on the shared
+// sub-sample grid; `sbuf` is the group's LDS scratch of kG * kRoundsMax
+// floats.  Returns true when the walk is over (tail cut, end of the window,
+// or the decision deferred to the fix-up kernel); `adv` is the number of
+// pixels the step covered.
 template <int kG, bool kLds, bool kStats>
 __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *sbuf, double *row,
                                  const float *gtab, int z0, int dir, int count, int c, int lane, LaneStats &st,
                                  long long k, int item, bool active, int &walked, int &adv)
 {
+    adv = 0;
+    if (!active) return false;
     const int glane = lane % kG;
     const double bintov = A.vbox / A.nbins;
-    const double tt = A.tautail;
     const double U0 = ((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
-    const bool far = fabs(U0) >= P.ufar;
-    StepShape sh;
-    sh.nm1 = far ? 0 : P.nm1;
-    sh.rounds = far ? 1 : P.rounds;
-    sh.pstep = far ? kG : P.pstep;
-    const int pcount = active ? min(sh.pstep, count - c) : 0;
+    if (fabs(U0) >= P.ufar) {
+        for (int r = 0; r < kFarRounds; ++r) {
+            const int cc = c + r * kG;
+            const int pc = min(kG, count - cc);
+            const bool valid = glane < pc;
+            const int z = z0 + dir * (cc + glane);
+            const double vlow = z * bintov - P.vel;
+            double t = 0;
+            if (valid) {
+                const double U = (vlow + 0.5 * bintov) * P.inv_b;
+                const float Y = P.ufar2 * __builtin_amdgcn_rcpf((float)(U * U));
+                float S = P.Cw[kWingTerms - 1];
+#pragma unroll
+                for (int q = kWingTerms - 2; q >= 0; --q) S = __builtin_fmaf(S, Y, P.Cw[q]);
+                t = P.scale * (double)(S * Y);
+                if (kStats) {
+                    st.evaluated++;
+                    st.far++;
+                }
+            }
+            if (settle<kG, kLds, kStats>(A, P, row, lane, valid, t, z, vlow, z0, dir, cc, pc, count, st, k, item,
+                                         walked))
+                return true;
+        }
+        adv = kFarRounds * kG;
+        return false;
+    }
+    const int nm1 = P.nm1;
+    const int pcount = min(P.pstep, count - c);
     wave_sync();   // the previous step's reads of sbuf are done
-    if (active && !far) {
-        const int nsub = sh.nm1 == 0 ? pcount : pcount * sh.nm1 + 1;
+    {
+        const int nsub = nm1 == 0 ? pcount : pcount * nm1 + 1;
         const int zlo = dir > 0 ? z0 + c : z0 - c - (pcount - 1);
-        for (int r = 0; r < sh.rounds; ++r) {
+        for (int r = 0; r < P.rounds; ++r) {
             const int m = glane + r * kG;
             if (m < nsub) {
                 double v;
-                if (sh.nm1 == 0) {
+                if (nm1 == 0) {
                     const double vlow = (z0 + dir * (c + m)) * bintov - P.vel;   // absorption.cpp:252
                     v = ((vlow + bintov) + vlow) / 2.;                           // singleabs.h:113
                 } else {
                     // sub-sample kk of pixel zlo + j, in increasing velocity
                     const int j = (int)(((float)m + 0.5f) * P.inv_nm1f);
-                    const int kk = m - j * sh.nm1;
+                    const int kk = m - j * nm1;
                     const double vlow = (zlo + j) * bintov - P.vel;
                     v = kk == 0 ? vlow : kk * P.dv + vlow;                       // singleabs.h:118-121
                 }
@@ -602,59 +676,23 @@ __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *
     const int z = z0 + dir * (c + glane);
     const double vlow = z * bintov - P.vel;
     double t = 0;
-    bool guard = false;
     if (valid) {
-        if (far) {
-            const double U = (vlow + 0.5 * bintov) * P.inv_b;
-            const float Y = P.ufar2 * __builtin_amdgcn_rcpf((float)(U * U));
-            float S = P.Cw[kWingTerms - 1];
-#pragma unroll
-            for (int q = kWingTerms - 2; q >= 0; --q) S = __builtin_fmaf(S, Y, P.Cw[q]);
-            t = P.scale * (double)(S * Y);
-        } else if (sh.nm1 == 0) {
+        if (nm1 == 0) {
             t = P.scale * (double)sbuf[glane];
         } else {
             // trapezoid over the pixel's sub-samples (singleabs.h:116-125)
             const int j = dir > 0 ? glane : pcount - 1 - glane;
-            const float *sp = sbuf + j * sh.nm1;
+            const float *sp = sbuf + j * nm1;
             double tot = 0.5 * (double)sp[0];
-            for (int kk = 1; kk < sh.nm1; ++kk) tot += (double)sp[kk];
-            tot += 0.5 * (double)sp[sh.nm1];
+            for (int kk = 1; kk < nm1; ++kk) tot += (double)sp[kk];
+            tot += 0.5 * (double)sp[nm1];
             t = P.scale * (tot * P.inv_nm1);
         }
-        if (kStats) {
-            st.evaluated++;
-            st.far += far;
-        }
-        guard = fabs(t - tt) <= kGuard * tt;
-    }
-    const unsigned long long mb = group_bits<kG>(__ballot(valid && !guard && t < tt), lane);
-    const unsigned long long mg = group_bits<kG>(__ballot(guard), lane);
-    const unsigned long long mall = mb | mg;
-    const int first = mall ? __builtin_ctzll(mall) : kG;
-    if (valid && (glane < first || (glane == first && ((mb >> glane) & 1ull)))) {
-        row_add<kLds>(&row[wrap_pixel(z, A.nbins)], t);
-        if (kStats) {
-            st.pixels++;
-            st.samples += pixel_samples(P, vlow, bintov);
-        }
+        if (kStats) st.evaluated++;
     }
     adv = pcount;
-    if (first < kG && ((mg >> first) & 1ull)) {
-        // the FP32 value of the deciding pixel is too close to tautail
-        if (active && glane == 0) push_fixup(A, k, item, z0 + dir * (c + first), count - (c + first), dir);
-        walked = -1;
-        return true;
-    }
-    if (mb != 0) {
-        walked = c + first + 1;
-        return true;
-    }
-    if (c + pcount >= count) {
-        walked = count;
-        return true;
-    }
-    return false;
+    return settle<kG, kLds, kStats>(A, P, row, lane, valid, t, z, vlow, z0, dir, c, pcount, count, st, k, item,
+                                    walked);
 }
 
 // Record the pixels walked in one direction of pair k (for the adjoint).
@@ -696,18 +734,18 @@ __device__ inline int walk64(const DepositArgs &A, PairTau &P, double *row, cons
 // 4 waves per SIMD: measured 8% faster at c1 than the 3 the register count
 // allows by default (a few dwords of spill in the pair setup).  kStats adds
 // the per-lane event counts (A.stats set).
-template <int kG, bool kLds, bool kStats>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void tau_kernel(DepositArgs A, const WorkItem *items, int item_base,
+template <int kG, bool kLds, bool kStats, int kBlock>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void tau_kernel(DepositArgs A, const WorkItem *items, int item_base,
                                                        int use_queue)
 {
-    constexpr int kGroups = kThreads / kG;
+    constexpr int kGroups = kBlock / kG;
     extern __shared__ double smem[];
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kGroups];
-    __shared__ float sub_sums[kThreads * kRoundsMax];
+    __shared__ float sub_sums[kBlock * kRoundsMax];
     __shared__ int s_item;
     __shared__ unsigned long long s_next;
-    for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
+    for (int i = threadIdx.x; i < kGTableSize; i += kBlock) gtab[i] = c_gtab[i];
     const int lane = threadIdx.x & 63;
     const int glane = lane % kG;
     const int gid = threadIdx.x / kG;
@@ -736,7 +774,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         double *out_row = L.out + (size_t)w.los * A.nbins;
         double *row = kLds ? smem : out_row;
         if (kLds)
-            for (int j = threadIdx.x; j < A.nbins; j += kThreads) row[j] = 0.0;
+            for (int j = threadIdx.x; j < A.nbins; j += kBlock) row[j] = 0.0;
         if (threadIdx.x == 0) s_next = (unsigned long long)w.p0;
         __syncthreads();
         const LosInfo los = los_info(A, w.los);
@@ -750,7 +788,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 kk = (unsigned long long)__shfl((long long)kk, gleader, 64);
                 if ((long long)kk >= w.p1) {
                     dir = 2;
-                } else if (setup_pair<kG>(A, L, los, A.pair_part[kk], P, glane, kG * kRoundsMax)) {
+                } else if (setup_pair<kG>(A, w.line, L, los, A.pair_part[kk], P, glane, kG * kRoundsMax)) {
                     k = (long long)kk;
                     if (kStats) st.pairs += (glane == 0);
                     if (P.fp64) {
@@ -789,9 +827,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         if (kLds) {
             __syncthreads();
             if (w.direct) {
-                for (int j = threadIdx.x; j < A.nbins; j += kThreads) out_row[j] = row[j];
+                for (int j = threadIdx.x; j < A.nbins; j += kBlock) out_row[j] = row[j];
             } else {
-                for (int j = threadIdx.x; j < A.nbins; j += kThreads) {
+                for (int j = threadIdx.x; j < A.nbins; j += kBlock) {
                     const double v = row[j];
                     if (v != 0.0) unsafeAtomicAdd(&out_row[j], v);
                 }
@@ -828,7 +866,7 @@ __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, cons
         const LineDesc L = line_of(A, w.line);
         double *out_row = L.out + (size_t)w.los * A.nbins;
         const LosInfo los = los_info(A, w.los);
-        if (!setup_pair<64>(A, L, los, A.pair_part[f.k], P, lane)) continue;
+        if (!setup_pair<64>(A, w.line, L, los, A.pair_part[f.k], P, lane)) continue;
         if (f.dir == 0) {
             const int lf = walk64<kWalkFull64>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k, f.item);
             const int lb = walk64<kWalkFull64>(A, P, out_row, gtab, P.zmax - 1, -1, half, lane, st, f.k, f.item);
@@ -979,7 +1017,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             if (!__ballot(have)) break;
             if (!have) continue;
             const long long k = (long long)kk;
-            if (!setup_pair<kG>(A, L, los, A.pair_part[k], P, glane)) continue;
+            if (!setup_pair<kG>(A, w.line, L, los, A.pair_part[k], P, glane)) continue;
             const long long wi = ((long long)w.line * A.npairs + k) * 2;
             const int lf = A.win[wi], lb = A.win[wi + 1];
             const int n = lf + lb;
@@ -1177,40 +1215,57 @@ static int lds_limit(const void *fn, size_t shm, size_t &set_to)
     return FSAMD_OK;
 }
 
-template <int kG, bool kLds, bool kStats>
+template <int kG, bool kLds, bool kStats, int kBlock>
 static int launch_tau_k(const DepositArgs &A, const WorkItem *items, int item_base, int nblocks, int use_queue,
                         size_t shm, hipStream_t s)
 {
     static size_t set_to = 0;
     int rc;
-    if (kLds && (rc = lds_limit((const void *)tau_kernel<kG, kLds, kStats>, shm, set_to))) return rc;
-    tau_kernel<kG, kLds, kStats><<<nblocks, kThreads, shm, s>>>(A, items, item_base, use_queue);
+    if (kLds && (rc = lds_limit((const void *)tau_kernel<kG, kLds, kStats, kBlock>, shm, set_to))) return rc;
+    tau_kernel<kG, kLds, kStats, kBlock><<<nblocks, kBlock, shm, s>>>(A, items, item_base, use_queue);
     return FSAMD_OK;
+}
+
+// Wavefronts per block (per sightline work item) of the tau kernel: 2, 4 or
+// 8 with the row in LDS; the counting variant and the global-row variant
+// are built for 4 only.
+int effective_waves(bool lds, bool stats, int waves)
+{
+    if (stats || !lds) return 4;
+    return waves == 2 || waves == 8 ? waves : 4;
 }
 
 template <int kG, bool kLds>
 static int launch_tau(const DepositArgs &A, const WorkItem *items, int item_base, int nblocks, int use_queue,
-                      size_t shm, hipStream_t s)
+                      size_t shm, int waves, hipStream_t s)
 {
-    return A.stats ? launch_tau_k<kG, kLds, true>(A, items, item_base, nblocks, use_queue, shm, s)
-                   : launch_tau_k<kG, kLds, false>(A, items, item_base, nblocks, use_queue, shm, s);
+    if (A.stats) return launch_tau_k<kG, kLds, true, 256>(A, items, item_base, nblocks, use_queue, shm, s);
+    if constexpr (!kLds) {
+        return launch_tau_k<kG, kLds, false, 256>(A, items, item_base, nblocks, use_queue, shm, s);
+    } else {
+        switch (effective_waves(kLds, false, waves)) {
+        case 2: return launch_tau_k<kG, kLds, false, 128>(A, items, item_base, nblocks, use_queue, shm, s);
+        case 8: return launch_tau_k<kG, kLds, false, 512>(A, items, item_base, nblocks, use_queue, shm, s);
+        default: return launch_tau_k<kG, kLds, false, 256>(A, items, item_base, nblocks, use_queue, shm, s);
+        }
+    }
 }
 
-int launch_deposit(bool tau, bool lds, int gsize, const DepositArgs &A, const WorkItem *items, int item_base,
-                   int nblocks, int use_queue, hipStream_t s)
+int launch_deposit(bool tau, bool lds, int gsize, int waves, const DepositArgs &A, const WorkItem *items,
+                   int item_base, int nblocks, int use_queue, hipStream_t s)
 {
     const size_t shm = lds ? deposit_lds_bytes(A.nbins) : 0;
     int rc = FSAMD_OK;
     if (tau) {
         if (gsize == 16)
-            rc = lds ? launch_tau<16, true>(A, items, item_base, nblocks, use_queue, shm, s)
-                     : launch_tau<16, false>(A, items, item_base, nblocks, use_queue, shm, s);
+            rc = lds ? launch_tau<16, true>(A, items, item_base, nblocks, use_queue, shm, waves, s)
+                     : launch_tau<16, false>(A, items, item_base, nblocks, use_queue, shm, waves, s);
         else if (gsize == 32)
-            rc = lds ? launch_tau<32, true>(A, items, item_base, nblocks, use_queue, shm, s)
-                     : launch_tau<32, false>(A, items, item_base, nblocks, use_queue, shm, s);
+            rc = lds ? launch_tau<32, true>(A, items, item_base, nblocks, use_queue, shm, waves, s)
+                     : launch_tau<32, false>(A, items, item_base, nblocks, use_queue, shm, waves, s);
         else
-            rc = lds ? launch_tau<64, true>(A, items, item_base, nblocks, use_queue, shm, s)
-                     : launch_tau<64, false>(A, items, item_base, nblocks, use_queue, shm, s);
+            rc = lds ? launch_tau<64, true>(A, items, item_base, nblocks, use_queue, shm, waves, s)
+                     : launch_tau<64, false>(A, items, item_base, nblocks, use_queue, shm, waves, s);
         if (rc) return rc;
     } else {
         if (lds) {
@@ -1240,12 +1295,13 @@ int launch_voigt_peak(long long per_thread, int nblocks, double du, double umax,
     return FSAMD_OK;
 }
 
-template <int kG>
-static int occ_tau(bool lds, size_t shm)
+template <int kG, int kBlock>
+static int occ_tau_k(bool lds, size_t shm)
 {
     int nb = 0;
-    hipError_t e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, true, false>, kThreads, shm)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, false, false>, kThreads, 0);
+    hipError_t e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, true, false, kBlock>, kBlock, shm)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tau_kernel<kG, false, false, 256>, 256, 0);
+    // (the global-row variant is built with 256 threads only: effective_waves)
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return 1;
@@ -1253,12 +1309,23 @@ static int occ_tau(bool lds, size_t shm)
     return nb;
 }
 
-int deposit_blocks_per_cu(bool tau, bool lds, int gsize, int nbins)
+template <int kG>
+static int occ_tau(bool lds, size_t shm, int waves)
+{
+    switch (effective_waves(lds, false, waves)) {
+    case 2: return occ_tau_k<kG, 128>(lds, shm);
+    case 8: return occ_tau_k<kG, 512>(lds, shm);
+    default: return occ_tau_k<kG, 256>(lds, shm);
+    }
+}
+
+int deposit_blocks_per_cu(bool tau, bool lds, int gsize, int waves, int nbins)
 {
     int nb = 0;
     const size_t shm = lds ? deposit_lds_bytes(nbins) : 0;
     if (tau) {
-        nb = gsize == 16 ? occ_tau<16>(lds, shm) : gsize == 32 ? occ_tau<32>(lds, shm) : occ_tau<64>(lds, shm);
+        nb = gsize == 16 ? occ_tau<16>(lds, shm, waves)
+             : gsize == 32 ? occ_tau<32>(lds, shm, waves) : occ_tau<64>(lds, shm, waves);
     } else {
         hipError_t e = lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, colden_kernel<true>, kThreads, shm)
                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, colden_kernel<false>, kThreads, 0);

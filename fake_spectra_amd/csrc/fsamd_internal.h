@@ -68,6 +68,17 @@ struct LineDesc {
 
 constexpr int kMaxLines = 8;   // lines per deposit launch
 
+// One particle's inputs to the optical-depth kernels in one 64-byte record
+// (a single cache line per pair instead of one per input array): position,
+// support radius, velocity, temperature and the density of each line of the
+// launch.  Packed once per launch from the caller's arrays.
+struct PartRec {
+    float x, y, z, h;
+    float vx, vy, vz, temp;
+    float dens[kMaxLines];
+};
+static_assert(sizeof(PartRec) == 64, "PartRec is one 64-byte record");
+
 // A walk handed from the FP32 deposit kernel to the FP64-capable fix-up
 // kernel: pixels z0, z0 + dir, ... (count of them) of pair k in work item
 // `item`; dir == 0 means the whole pair (both directions) in FP64.
@@ -88,6 +99,7 @@ struct DepositArgs {
     double vbox;       // box * velfac
     double box;
     double tautail;
+    const PartRec *rec; // [N] packed inputs (tau and adjoint kernels)
     const float *pos;   // [N,3]
     const float *vel;   // [N,3]
     const float *temp;  // [N]
@@ -114,6 +126,8 @@ struct DepositArgs {
 // Host-side launchers (return FSAMD_OK or an error code).
 int launch_fill_i32(int *p, int value, long long n, hipStream_t s);
 int launch_fill_f64(double *p, double value, long long n, hipStream_t s);
+int launch_pack_records(const float *pos, const float *vel, const float *temp, const float *h,
+                        const LineDesc *lines, int nlines, long long n, PartRec *out, hipStream_t s);
 int exclusive_scan_i32_to_i64(const int *in, long long *out, long long n, void *ws, size_t ws_bytes,
                               hipStream_t s);
 size_t exclusive_scan_ws_bytes(long long n);

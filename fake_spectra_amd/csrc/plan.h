@@ -81,6 +81,7 @@ struct fsamd_plan {
     LineTable eager_lines;
     std::vector<std::unique_ptr<LineTable>> pinned_lines;
     fsamd::DevBuf stats;        // kNStats counters
+    fsamd::DevBuf rec;          // [npart] packed particle records of the current launch
     fsamd::DevBuf fix;          // fix-up list (FP64 walks)
     fsamd::DevBuf fix_count;    // [count, overflow] per line group
     int fix_cap = 0;
@@ -91,6 +92,7 @@ struct fsamd_plan {
     int win_lines = 0, win_nbins = 0, win_kernel = -1;
     // lane-group width chosen by fsamd_tau_autotune for (nbins, nlines, kernel)
     int tuned_g = 0;
+    int tuned_waves = 4;
     long long tuned_key = -1;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // call, deposit-kernel brackets
     // Once a call on this plan has been captured into a graph, buffers that

@@ -196,20 +196,24 @@ def tau_adjoint(plan: Plan, nbins: int, kernel: int, velfac: float, vel: torch.T
 
 def autotune(plan: Plan, nbins: int, kernel: int, velfac: float, vel: torch.Tensor, temp: torch.Tensor,
              lines: Sequence, dens: Sequence[torch.Tensor], tautail: float = 1e-7):
-    """Pick the tau kernel's lane-group width for this plan and call shape by
-    timing 16, 32 and 64 on a sample of the work; returns (width, {width: ms})."""
+    """Pick the tau kernel's lane-group width and wavefronts per sightline
+    for this plan and call shape by timing them on a sample of the work;
+    returns (width, waves, {(width, waves): ms})."""
     vel = _require(vel, torch.float32, "vel")
     temp = _require(temp, torch.float32, "temp")
     quads = [_as_quad(x) for x in lines]
     dens = [_require(d, torch.float32, "dens") for d in dens]
     scratch = [torch.empty((plan.nlos, nbins), dtype=torch.float64, device=plan.pos.device) for _ in quads]
     arr = _line_structs(quads, dens, scratch)
-    best = C.c_int()
-    ms = (C.c_float * 3)()
+    best, waves = C.c_int(), C.c_int()
+    ms = (C.c_float * 5)()
     N.check(N.lib().fsamd_tau_autotune(plan.handle, int(nbins), int(kernel), float(velfac), float(tautail),
                                        _ptr(plan.pos), _ptr(vel), _ptr(temp), _ptr(plan.h), arr, len(quads), _stream(),
-                                       C.byref(best), ms))
-    return best.value, {16: ms[0], 32: ms[1], 64: ms[2]}
+                                       C.byref(best), C.byref(waves), ms))
+    times = {(16, 4): ms[0], (32, 4): ms[1], (64, 4): ms[2]}
+    times[(best.value, 2)] = ms[3]    # the wavefront counts are timed at the best width
+    times[(best.value, 8)] = ms[4]
+    return best.value, waves.value, times
 
 
 class TauGraph:

@@ -134,14 +134,16 @@ double fsamd_last_ms(void);
 double fsamd_last_kernel_ms(void);
 
 /* Autotuner (no counterpart in the reference): times the batched tau launch
- * for lane-group widths 16, 32 and 64 on every k-th sightline of the plan's
- * heaviest-first order (k = nlos / 16384, all sightlines below 32768) and keeps the
- * fastest for later fsamd_tau calls with the same nbins, line count and
- * kernel.  Writes partial results into the lines' outputs.  ms3 (optional,
- * host) receives the three times. */
+ * on every k-th sightline of the plan's heaviest-first order (k = nlos /
+ * 16384, all sightlines below 32768) for lane-group widths 16, 32 and 64 at 4
+ * wavefronts per sightline work item, then 2 and 8 wavefronts at the best
+ * width, and keeps the fastest pair for later fsamd_tau calls with the same
+ * nbins, line count and kernel.  Writes partial results into the lines'
+ * outputs.  ms5 (optional, host) receives the five times in that order. */
 int fsamd_tau_autotune(fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail,
                        const float *pos, const float *vel, const float *temp, const float *h,
-                       const fsamd_line *lines, int nlines, void *stream, int *best_g, float *ms3);
+                       const fsamd_line *lines, int nlines, void *stream, int *best_g, int *best_waves,
+                       float *ms5);
 
 /* fsamd_tau inside a hipGraph capture (the stream reports capturing):
  * allowed in the batched mode without statistics after one uncaptured call
@@ -155,6 +157,11 @@ int fsamd_plan_overflow(fsamd_plan *plan, int *flag, void *stream);
  * group of the tau kernel (16, 32 or 64; a 64-lane wavefront walks 64/g
  * (pair, direction) streams at once).  Process-wide; default 16. */
 int fsamd_set_group_size(int g);
+
+/* Tuning knob with no counterpart in the reference: wavefronts per block of
+ * the tau kernel, i.e. per (sightline, line) work item (2, 4 or 8; the
+ * counting variant runs with 4).  Process-wide; default 4. */
+int fsamd_set_block_waves(int w);
 
 /* Diagnostic with no counterpart in the reference: isolated throughput of the
  * FP32 sample evaluator (kernel weight x Voigt profile) in samples/s, over

@@ -104,20 +104,24 @@ def test_tau_configs_vs_oracle(oracle, idx, scale, nlos):
     assert_colden_close(col.cpu().numpy(), oracle.interpolate(*w.interp_args(None, False)), "c%d colden" % idx)
 
 
-@pytest.mark.parametrize("g", [16, 32, 64])
-def test_group_sizes_match_oracle(oracle, g):
-    """Every lane-group width of the tau kernel gives the same spectra."""
+@pytest.mark.parametrize("g,waves", [(16, 4), (32, 4), (64, 4), (16, 2), (32, 8)])
+def test_group_sizes_match_oracle(oracle, g, waves):
+    """Every lane-group width and wavefront count of the tau kernel gives
+    the same spectra."""
     from fake_spectra_amd import _native as N
     w = synth.config(1, scale=1 / 64, nlos=8)
     plan = plan_for(w)
     N.set_group_size(g)
+    N.set_block_waves(waves)
     try:
         outs = SP.tau(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), w.lines,
                       [dev(w.dens_for(ln)) for ln in w.lines], w.tautail)
     finally:
         N.set_group_size(16)
+        N.set_block_waves(4)
     for ln, o in zip(w.lines, outs):
-        assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)), "G=%d %s" % (g, ln.name))
+        assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)),
+                         "G=%d waves=%d %s" % (g, waves, ln.name))
 
 
 def test_graph_replays_match_eager():
@@ -147,8 +151,9 @@ def test_autotune_picks_a_width(oracle):
     plan = plan_for(w)
     vel, temp = dev(w.vel), dev(w.temp)
     dens = [dev(w.dens_for(ln)) for ln in w.lines]
-    best, ms = SP.autotune(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
-    assert best in (16, 32, 64) and all(v > 0 for v in ms.values())
+    best, waves, ms = SP.autotune(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
+    assert best in (16, 32, 64) and waves in (2, 4, 8) and all(v > 0 for v in ms.values())
+    assert len(ms) == 5
     outs = SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
     for ln, o in zip(w.lines, outs):
         assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)), "tuned " + ln.name)
