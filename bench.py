@@ -9,6 +9,9 @@ in one batched launch.  The default workload is 256^3 particles in a
 (synthetic gas field, seeded).  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--config 1|2|...] [--no-cpu] [--no-plain]
+
+Defaults: 3 untimed warm-up steps, 10 timed steps (value from their total
+time; the median step is reported beside it).
 """
 from __future__ import annotations
 
@@ -35,8 +38,8 @@ T_START = time.perf_counter()
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -213,6 +216,7 @@ def main():
     state = {}
 
     def step():
+        ts = time.perf_counter()
         plan = SP.Plan(w.box, pos, hh, ax, cofm)
         SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail, out=outs)
         st = SP.last_stats()
@@ -221,6 +225,7 @@ def main():
         state.setdefault("kernel_ms", []).append(st["kernel_ms"])
         state["stats"] = st
         plan.close()
+        state.setdefault("wall", []).append(time.perf_counter() - ts)   # the tau call synchronized
 
     # timed steps carry only hipEvents around the deposit; the counters come
     # from one more, untimed step
@@ -229,12 +234,14 @@ def main():
         step()
     state["ms"] = []
     state["kernel_ms"] = []
+    state["wall"] = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    step_median_ms = 1e3 * float(np.median(state["wall"]))
     dep_ms = float(np.mean(state["ms"]))              # deposit call: fills + tau kernel + FP64 fix-up
     ker_ms = float(np.mean(state["kernel_ms"]))       # the tau kernel launches alone
     progress("timed steps: %.1f ms per step" % (1e3 * elapsed / args.steps))
@@ -292,6 +299,7 @@ def main():
                  "fp64_tailcut_pixels": st["fp64_pixels"], "fp64_pairs": st["fp64_pairs"],
                  "kernel_sums": st["kernel_sums"], "far_wing_pixels": st["far_pixels"]},
         "pairs": M,
+        "step_median_ms": step_median_ms,
         "deposit_ms": dep_ms,
         "binning_ms": ms_per_step - dep_ms,
     }

@@ -491,9 +491,12 @@ __device__ inline unsigned long long group_bits(unsigned long long m, int lane)
     return (m >> ((lane / kG) * kG)) & ((1ull << kG) - 1ull);
 }
 
-// One step of kG consecutive pixels of a walk (lane glane of the group);
-// returns true when the walk is over (tail cut reached, end of the window, or
-// the decision deferred to the fix-up kernel).
+// One step of kG consecutive pixels of a walk (lane glane of the group), each
+// pixel evaluated on its own as tau_kern_outer does (singleabs.h:104-126,
+// sub-sample grid from the pixel's own vhigh - vlow): the fix-up kernel's
+// walk (walk64), in the guarded or the all-FP64 mode.  Returns true when the
+// walk is over (tail cut reached, end of the window, or in kWalkMain mode the
+// decision deferred).
 template <int kG, int kMode, bool kLds, bool kStats>
 __device__ inline bool walk_chunk(const DepositArgs &A, const PairTau &P, double *row, const float *gtab, int z0,
                                   int dir, int count, int c, int lane, LaneStats &st, long long k, int item,
