@@ -178,6 +178,17 @@ def colden_leg(w, pos, hh, ax, cofm, dens, steps):
             "note": "binning + batched colden launch"}
 
 
+def cyc_split(st):
+    """Fractions of the counting kernel's wavefront cycles in pair set-up,
+    far-wing set-up and walk steps, and the share of wave iterations that set
+    up at least one pair."""
+    tot = st["cycles_setup"] + st["cycles_wing"] + st["cycles_walk"]
+    if tot <= 0 or st["wave_iterations"] <= 0:
+        return None
+    return {"setup": st["cycles_setup"] / tot, "wing": st["cycles_wing"] / tot, "walk": st["cycles_walk"] / tot,
+            "iterations_with_setup": st["setup_iterations"] / st["wave_iterations"]}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -297,7 +308,8 @@ def main():
                  "voigt_samples_per_launch": st["voigt_samples"],
                  "pixels_deposited": st["pixels"], "pixels_evaluated": st["evaluated_pixels"],
                  "fp64_tailcut_pixels": st["fp64_pixels"], "fp64_pairs": st["fp64_pairs"],
-                 "kernel_sums": st["kernel_sums"], "far_wing_pixels": st["far_pixels"]},
+                 "kernel_sums": st["kernel_sums"], "far_wing_pixels": st["far_pixels"],
+                 "wave_cycles": cyc_split(st)},
         "pairs": M,
         "step_median_ms": step_median_ms,
         "deposit_ms": dep_ms,

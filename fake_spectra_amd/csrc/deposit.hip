@@ -30,6 +30,7 @@
 #include "plan.h"
 #include "voigt.h"
 #include "deposit.h"
+#include "wing_table.h"
 
 namespace fsamd {
 
@@ -50,6 +51,10 @@ __constant__ double c_wing_p[kWingTerms] = {1.0, 1.5, 3.75, 13.125, 59.0625, 324
                                              15836.1328125, 134607.12890625, 1278767.724609375};
 __constant__ double c_wing_q[kWingTerms] = {1.0, 5.0, 26.25, 157.5, 1082.8125, 8445.9375, 73901.953125,
                                              717904.6875, 7672606.34765625, 89513740.72265625};
+static_assert(FSAMD_WING_NTERMS == kWingTerms, "wing_table.h");
+__constant__ double c_wing_T[(FSAMD_WING_TMAX + 1) * kWingTerms] = FSAMD_WING_T_INIT;
+__constant__ double c_binom_even[kWingTerms * kWingTerms] = FSAMD_BINOM_EVEN_INIT;
+__constant__ double c_binom_odd[kWingTerms * kWingTerms] = FSAMD_BINOM_ODD_INIT;
 
 // singleabs.h:17-26
 __device__ inline double sph_cubic_kernel(double q)
@@ -255,6 +260,18 @@ __device__ inline double taulast_pixel(const PairTau &P, double vlow, double bin
     return P.scale * (total / (np - 1));
 }
 
+// x^n for 0 <= n < 16 by squaring, with a uniform instruction stream.
+__device__ inline double ipow(double x, int n)
+{
+    double r = 1.0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+        if (n & (1 << bit)) r *= x;
+        x *= x;
+    }
+    return r;
+}
+
 // Far-wing coefficients of one pair.  The sample offsets from the pixel
 // centre are c = x_i - y_j with x_i = vv_i/b the kernel samples (weights W_i)
 // and y_j the sub-sample offsets of tau_kern_outer (singleabs.h:110-125,
@@ -263,62 +280,63 @@ __device__ inline double taulast_pixel(const PairTau &P, double vlow, double bin
 // Y_k = sum t_j y_j^2k.  Offsets are measured in units of ufar so that all
 // coefficients stay within FP32 range:  C'_n = sum_(k+j=n) g_k ufar^(-2k-2)
 // C(2n+1,2j) m'_j with m'_j the moments of c/ufar.  Lane n < 10 builds the
-// n-th moments and coefficient and stores C'_n.
+// n-th moments and coefficient and stores C'_n.  The kernel samples are
+// x_i = (vhigh/b)(i-3)/4 (singleabs.h:146-149), so X_k takes the three
+// positive offsets with the summed weights of each +- pair; Y_k = h^2k T[m][k]
+// for a pixel of half-width h in np = m + 1 sub-samples, with T tabulated
+// exactly (wing_table.h) up to m = FSAMD_WING_TMAX.
 template <int kG, bool kAdj = false>
 __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
 {
     const double inv_u = 1.0 / P.ufar;
     const int np = P.np;
+    const int m1 = np - 1;
     const int n = lane < kWingTerms ? lane : 0;
-    // X_n: kernel samples
-    double Xn = 0.0;
-#pragma unroll 1
-    for (int i = 0; i < kNGrid - 1; ++i) {
-        const double x2 = (P.cv[i] * inv_u) * (P.cv[i] * inv_u);
-        double pw = 1.0;
-        for (int r = 0; r < n; ++r) pw *= x2;
-        Xn += (double)P.W[i] * pw;
-    }
     // Y_n: pixel sub-samples
-    double Yn = 1.0;
-    if (np > 1) {
-        const double dv = bintov / (np - 1);
+    double Yn;
+    if (m1 == 0) {
+        Yn = n == 0 ? 1.0 : 0.0;
+    } else if (m1 <= FSAMD_WING_TMAX) {
+        const double h = 0.5 * bintov * P.inv_b * inv_u;
+        Yn = ipow(h * h, n) * c_wing_T[m1 * kWingTerms + n];
+    } else {
+        const double dv = bintov / m1;
         Yn = 0.0;
         for (int j = 0; j < np; ++j) {
             const double y = (j * dv - 0.5 * bintov) * P.inv_b * inv_u;
-            const double t = ((j == 0 || j == np - 1) ? 0.5 : 1.0) / (np - 1);
-            double pw = 1.0;
-            for (int r = 0; r < n; ++r) pw *= y * y;
-            Yn += t * pw;
+            const double t = ((j == 0 || j == m1) ? 0.5 : 1.0) / m1;
+            Yn += t * ipow(y * y, n);
         }
-    } else if (n > 0) {
-        Yn = 0.0;
     }
-    // m'_n = sum_k C(2n,2k) X_k Y_(n-k)
-    double m = 0.0, bin = 1.0;
+    // m'_n = sum_k C(2n,2k) X_k Y_(n-k), X_k by running powers
+    const double x4 = (P.cv[4] * inv_u) * (P.cv[4] * inv_u);
+    const double x5 = (P.cv[5] * inv_u) * (P.cv[5] * inv_u);
+    const double x6 = (P.cv[6] * inv_u) * (P.cv[6] * inv_u);
+    const double w4 = (double)P.W[4] + (double)P.W[2];
+    const double w5 = (double)P.W[5] + (double)P.W[1];
+    const double w6 = (double)P.W[6] + (double)P.W[0];
+    double p4 = 1.0, p5 = 1.0, p6 = 1.0, mom = 0.0;
 #pragma unroll 1
     for (int k = 0; k < kWingTerms; ++k) {
-        const double xk = shfl_d<kG>(Xn, k);
+        double xk = w4 * p4 + w5 * p5 + w6 * p6;
+        if (k == 0) xk += (double)P.W[3];
+        p4 *= x4;
+        p5 *= x5;
+        p6 *= x6;
         const double yk = shfl_d<kG>(Yn, n - k < 0 ? 0 : n - k);
-        if (k > 0) bin = bin * (double)(2 * n - 2 * k + 2) * (double)(2 * n - 2 * k + 1) /
-                         ((double)(2 * k - 1) * (double)(2 * k));
-        if (k <= n) m += bin * xk * yk;
+        if (k <= n) mom += c_binom_even[n * kWingTerms + k] * xk * yk;
     }
     // C'_n = sum_(k+j=n) g_k ufar^(-2k-2) C(2n+1, 2j) m'_j
-    const double a = P.aa, a2 = a * a, asq = a / kSqrtPi, iu2 = inv_u * inv_u;
-    double C = 0.0, CF = 0.0, binom = 1.0, gs = asq * iu2;
-    // walk j = 0..n (k = n..0): start from k = n, so scale ufar^(-2n-2)
-    for (int r = 0; r < n; ++r) gs *= iu2;
+    const double a = P.aa, a2 = a * a, iu2 = inv_u * inv_u, u2 = P.ufar * P.ufar;
+    // j = 0..n walks k = n..0: start from ufar^(-2n-2)
+    double C = 0.0, CF = 0.0, gs = a * (1.0 / kSqrtPi) * iu2 * ipow(iu2, n);
 #pragma unroll 1
     for (int j = 0; j < kWingTerms; ++j) {
-        const double mj = shfl_d<kG>(m, j);
-        if (j > 0) {
-            binom = binom * (double)(2 * n + 3 - 2 * j) * (double)(2 * n + 2 - 2 * j) /
-                    ((double)(2 * j - 1) * (double)(2 * j));
-            gs *= P.ufar * P.ufar;
-        }
+        const double mj = shfl_d<kG>(mom, j);
+        if (j > 0) gs *= u2;
         if (j <= n) {
             const int k = n - j;
+            const double binom = c_binom_odd[n * kWingTerms + j];
             const double qk = k > 0 ? c_wing_q[k - 1] : 0.0;
             C += gs * (c_wing_p[k] - a2 * qk) * binom * mj;
             // u dH/du + a dH/da of the term a p_k u^(-2k-2) - a^3 q_(k-1) u^(-2k-2)
@@ -362,11 +380,44 @@ __device__ inline StepShape step_shape(int np, int kg)
     return sh;
 }
 
-// Set up one pair in the wave's LDS slot; returns false if the particle does
-// not reach the sightline.  Every lane of the wave calls it.
+__device__ inline void push_fixup(const DepositArgs &A, long long k, int item, int z0, int count, int dir)
+{
+    const int slot = atomicAdd(&A.fix_count[0], 1);
+    if (slot < A.fix_cap)
+        A.fix[slot] = Fixup{k, item, z0, count, dir};
+    else
+        atomicOr(&A.fix_count[1], 1);
+}
+
+// Lanes of a wave are split into groups of kG lanes; each group walks its own
+// (pair, direction) stream, kG consecutive pixels per step.
 template <int kG>
-__device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc &L, const LosInfo &los, int p,
-                                  PairTau &P, int lane, int max_sums = 1 << 30)
+__device__ inline unsigned long long group_bits(unsigned long long m, int lane)
+{
+    if (kG == 64) return m;
+    return (m >> ((lane / kG) * kG)) & ((1ull << kG) - 1ull);
+}
+
+struct LaneStats {
+    long long pairs = 0, pixels = 0, px64 = 0, pairs64 = 0, samples = 0, evaluated = 0, sums = 0, far = 0;
+    // wave cycles (s_memtime, counted by lane 0 of each wavefront) in pair
+    // set-up, far-wing set-up and walk steps, wave iterations, and the
+    // iterations in which at least one group set up a pair
+    long long cyc_setup = 0, cyc_wing = 0, cyc_walk = 0, iters = 0, setup_iters = 0;
+};
+
+// Per-pair scalars of LineAbsorption::add_tau_particle (absorption.cpp:218-247)
+// and SingleAbsorber's constructor (singleabs.h:81-90).
+struct PairConsts {
+    double vel, b, inv_b, dvk, vhigh, vdr2, vsmooth, aa, scale, scale_nd, ufar;
+    float temp;
+    int zmax, np, fp64;
+};
+
+// Returns false if the particle does not reach the sightline
+// (absorption.cpp:232).  fp64 routes the pair to the FP64 path.
+__device__ inline bool pair_consts(const DepositArgs &A, int line, const LineDesc &L, const LosInfo &los, int p,
+                                   int max_sums, PairConsts &C)
 {
     const float4 r0 = *(const float4 *)&A.rec[p].x;
     const float4 r1 = *(const float4 *)&A.rec[p].vx;
@@ -403,40 +454,71 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
     // FP64 path, which follows the reference pixel by pixel.
     const bool grid_edge = fabs(bintov - b / 2.) <= 1e-12 * bintov || (np > 1 && fabs(xs - rint(xs)) <= 1e-12 * xs);
     // pixels wider than max_sums sub-samples do not fit one walk step
-    const int fp64 = !(aa <= (double)kA32Max) || grid_edge || np - 1 > max_sums - 1;
+    C.fp64 = !(aa <= (double)kA32Max) || grid_edge || np - 1 > max_sums - 1;
     const double dmax = (vhigh - dvk) * inv_b + (np > 1 ? 0.5 * bintov * inv_b : 0.0);
-    const double ufar = fmax(7.0 + dmax, 3.0 * dmax);
+    C.ufar = fmax(7.0 + dmax, 3.0 * dmax);
+    C.vel = vel;
+    C.b = b;
+    C.inv_b = inv_b;
+    C.dvk = dvk;
+    C.vhigh = vhigh;
+    C.vdr2 = vdr2;
+    C.vsmooth = vsmooth;
+    C.aa = aa;
+    C.scale = amp * (double)dens * dvk * A.inv_velfac;
+    C.scale_nd = amp * dvk * A.inv_velfac;
+    C.temp = temp;
+    C.zmax = (int)floor(vel / bintov);
+    C.np = np;
+    return true;
+}
+
+// Kernel weight of interior sample i (singleabs.h:145-152).
+__device__ inline float sample_weight(const DepositArgs &A, const PairConsts &C, int i)
+{
+    const double vv = (i + 1) * C.dvk - C.vhigh;
+    const double q = sqrt(C.vdr2 + vv * vv) / C.vsmooth;
+    return (float)kernel_weight(A.kernel, q);
+}
+
+// Set up one pair in the wave's LDS slot; returns false if the particle does
+// not reach the sightline.  Every lane of the wave calls it.
+template <int kG>
+__device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc &L, const LosInfo &los, int p,
+                                  PairTau &P, int lane, int max_sums = 1 << 30)
+{
+    PairConsts C;
+    if (!pair_consts(A, line, L, los, p, max_sums, C)) return false;
+    const double bintov = A.vbox / A.nbins;
     wave_sync();   // the previous pair's constants are no longer read
     if (lane < kNGrid - 1) {
-        const double vv = (lane + 1) * dvk - vhigh;
-        const double q = sqrt(vdr2 + vv * vv) / vsmooth;
-        P.cv[lane] = vv * inv_b;
-        P.W[lane] = (float)kernel_weight(A.kernel, q);
+        P.cv[lane] = ((lane + 1) * C.dvk - C.vhigh) * C.inv_b;
+        P.W[lane] = sample_weight(A, C, lane);
     }
     if (lane == 0) {
-        P.vel = vel;
-        P.b = b;
-        P.inv_b = inv_b;
-        P.dvk = dvk;
-        P.vhigh = vhigh;
-        P.vdr2 = vdr2;
-        P.vsmooth = vsmooth;
-        P.aa = aa;
-        P.zmax = (int)floor(vel / bintov);
-        P.fp64 = fp64;
-        P.va = make_voigt_a(fp64 ? 0.0 : aa);
-        P.scale = amp * (double)dens * dvk * A.inv_velfac;
-        P.scale_nd = amp * dvk * A.inv_velfac;
-        P.temp = (double)temp;
+        P.vel = C.vel;
+        P.b = C.b;
+        P.inv_b = C.inv_b;
+        P.dvk = C.dvk;
+        P.vhigh = C.vhigh;
+        P.vdr2 = C.vdr2;
+        P.vsmooth = C.vsmooth;
+        P.aa = C.aa;
+        P.zmax = C.zmax;
+        P.fp64 = C.fp64;
+        P.va = make_voigt_a(C.fp64 ? 0.0 : C.aa);
+        P.scale = C.scale;
+        P.scale_nd = C.scale_nd;
+        P.temp = (double)C.temp;
         P.part = p;
-        P.np = np;
-        P.ufar = ufar;
-        P.ufar2 = (float)(ufar * ufar);
+        P.np = C.np;
+        P.ufar = C.ufar;
+        P.ufar2 = (float)(C.ufar * C.ufar);
         P.wing_ready = 0;
-        P.inv_nm1 = np > 1 ? 1.0 / (np - 1) : 1.0;
-        P.dv = np > 1 ? bintov * P.inv_nm1 : 0.0;
+        P.inv_nm1 = C.np > 1 ? 1.0 / (C.np - 1) : 1.0;
+        P.dv = C.np > 1 ? bintov * P.inv_nm1 : 0.0;
         P.inv_nm1f = (float)P.inv_nm1;
-        const StepShape sh = step_shape(np, kG);
+        const StepShape sh = step_shape(C.np, kG);
         P.nm1 = sh.nm1;
         P.rounds = sh.rounds;
         P.pstep = sh.pstep;
@@ -445,9 +527,114 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
     return true;
 }
 
-struct LaneStats {
-    long long pairs = 0, pixels = 0, px64 = 0, pairs64 = 0, samples = 0, evaluated = 0, sums = 0, far = 0;
+// The main kernel sets pairs up kQ at a time, one per lane, into a per-group
+// queue of compact records, so that the FP64 set-up arithmetic runs on kQ
+// lanes at once; activating a queued pair only expands its record into the
+// group's PairTau slot.
+constexpr int kQ = 8;
+
+struct PairLite {
+    double vel, inv_b, scale, vhigh, aa, inv_nm1;
+    long long k;      // pair index
+    float W[4];       // kernel weights of the centre sample and the 3 above it
+    float ufar;
+    int zmax;
+    int shape;        // nm1 | rounds << 8 | pstep << 16 (step_shape)
+    int pad;
 };
+static_assert(sizeof(PairLite) == 88, "PairLite layout");
+
+// Claim the next A.claim (<= kQ) pairs of item w for the group, set up one per lane, and
+// queue those walked in FP32; FP64 pairs go straight to the fix-up list.
+// Returns the number queued; `more` says whether the item has pairs left.
+template <int kG, bool kStats>
+__device__ inline int refill_pairs(const DepositArgs &A, const WorkItem &w, int gitem, const LineDesc &L,
+                                   const LosInfo &los, unsigned long long *next, PairLite *q, int lane, bool &more,
+                                   LaneStats &st)
+{
+    const int glane = lane % kG;
+    const int gleader = (lane / kG) * kG;
+    const int claim = A.claim < 1 ? 1 : (A.claim > kQ ? kQ : A.claim);
+    unsigned long long base = 0;
+    if (glane == 0) base = atomicAdd(next, (unsigned long long)claim);
+    base = (unsigned long long)__shfl((long long)base, gleader, 64);
+    const long long kk = (long long)base + glane;
+    bool enq = false;
+    PairConsts C;
+    if (glane < claim && kk < w.p1 && pair_consts(A, w.line, L, los, A.pair_part[kk], kG * kRoundsMax, C)) {
+        if (kStats) st.pairs++;
+        if (C.fp64) {
+            if (kStats) st.pairs64++;
+            push_fixup(A, kk, gitem, C.zmax, A.nbins / 2, 0);
+        } else {
+            enq = true;
+        }
+    }
+    const unsigned long long m = group_bits<kG>(__ballot(enq), lane);
+    wave_sync();   // the previous batch has been read
+    if (enq) {
+        PairLite r;
+        r.vel = C.vel;
+        r.inv_b = C.inv_b;
+        r.scale = C.scale;
+        r.vhigh = C.vhigh;
+        r.aa = C.aa;
+        r.inv_nm1 = C.np > 1 ? 1.0 / (C.np - 1) : 1.0;
+        r.k = kk;
+        const double inv_vs = 1.0 / C.vsmooth;
+        for (int i = 0; i < 4; ++i) {
+            const double vv = (i + 4) * C.dvk - C.vhigh;
+            r.W[i] = (float)kernel_weight(A.kernel, sqrt(C.vdr2 + vv * vv) * inv_vs);
+        }
+        r.ufar = (float)C.ufar;
+        r.zmax = C.zmax;
+        const StepShape sh = step_shape(C.np, kG);
+        r.shape = sh.nm1 | (sh.rounds << 8) | (sh.pstep << 16);
+        r.pad = 0;
+        q[__popcll(m & ((1ull << glane) - 1ull))] = r;
+    }
+    wave_sync();
+    more = (long long)base + claim < w.p1;
+    return __popcll(m);
+}
+
+// Expand queued record r into the group's PairTau slot (the fields the main
+// kernel's walk reads; setup_pair fills the same values).
+template <bool kStats>
+__device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane, double bintov)
+{
+    wave_sync();   // the previous pair's constants are no longer read
+    const double dvk = 2. * r.vhigh / kNGrid;
+    if (glane < kNGrid - 1) {
+        P.cv[glane] = ((glane + 1) * dvk - r.vhigh) * r.inv_b;
+        P.W[glane] = r.W[glane < 3 ? 3 - glane : glane - 3];   // K(q) is even in the sample offset
+    }
+    if (glane == 0) {
+        const int nm1 = r.shape & 0xff;
+        P.vel = r.vel;
+        P.inv_b = r.inv_b;
+        if (kStats) P.b = 1.0 / r.inv_b;
+        P.dvk = dvk;
+        P.vhigh = r.vhigh;
+        P.aa = r.aa;
+        P.zmax = r.zmax;
+        P.fp64 = 0;
+        P.va = make_voigt_a(r.aa);
+        P.scale = r.scale;
+        P.np = nm1 + 1;
+        P.ufar = (double)r.ufar;
+        P.ufar2 = (float)((double)r.ufar * (double)r.ufar);
+        P.wing_ready = 0;
+        P.inv_nm1 = r.inv_nm1;
+        P.dv = nm1 > 0 ? bintov * r.inv_nm1 : 0.0;
+        P.inv_nm1f = (float)r.inv_nm1;
+        P.nm1 = nm1;
+        P.rounds = (r.shape >> 8) & 0xff;
+        P.pstep = r.shape >> 16;
+    }
+    wave_sync();
+}
+
 
 // Voigt samples the reference evaluates for one pixel (singleabs.h:110-125).
 __device__ inline int pixel_samples(const PairTau &P, double vlow, double bintov)
@@ -473,23 +660,7 @@ __device__ inline void row_add(double *p, double v)
 // evaluates whole pairs in FP64.
 enum { kWalkMain = 0, kWalkGuarded = 1, kWalkFull64 = 2 };
 
-__device__ inline void push_fixup(const DepositArgs &A, long long k, int item, int z0, int count, int dir)
-{
-    const int slot = atomicAdd(&A.fix_count[0], 1);
-    if (slot < A.fix_cap)
-        A.fix[slot] = Fixup{k, item, z0, count, dir};
-    else
-        atomicOr(&A.fix_count[1], 1);
-}
 
-// Lanes of a wave are split into groups of kG lanes; each group walks its own
-// (pair, direction) stream, kG consecutive pixels per step.
-template <int kG>
-__device__ inline unsigned long long group_bits(unsigned long long m, int lane)
-{
-    if (kG == 64) return m;
-    return (m >> ((lane / kG) * kG)) & ((1ull << kG) - 1ull);
-}
 
 // One step of kG consecutive pixels of a walk (lane glane of the group), each
 // pixel evaluated on its own as tau_kern_outer does (singleabs.h:104-126,
@@ -746,15 +917,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kGroups];
     __shared__ float sub_sums[kBlock * kRoundsMax];
+    __shared__ PairLite pqueue[kGroups * kQ];
     __shared__ int s_item;
     __shared__ unsigned long long s_next;
     for (int i = threadIdx.x; i < kGTableSize; i += kBlock) gtab[i] = c_gtab[i];
     const int lane = threadIdx.x & 63;
     const int glane = lane % kG;
     const int gid = threadIdx.x / kG;
-    const int gleader = (lane / kG) * kG;
     PairTau &P = pslot[gid];
     float *sbuf = sub_sums + gid * kG * kRoundsMax;
+    PairLite *q = pqueue + gid * kQ;
     const double bintov = A.vbox / A.nbins;
     const int half = A.nbins / 2;
     LaneStats st;
@@ -784,36 +956,54 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         // group state: dir 0 = needs a pair, +1/-1 = walking, 2 = no pairs left
         long long k = 0;
         int dir = 0, c = 0, z0 = 0;
+        int qi = 0, qn = 0;      // the group's pair queue
+        bool more = true;
         for (;;) {
+            long long tc0 = 0;
+            if (kStats) {
+                tc0 = clock64();
+                const bool any = __ballot(dir == 0) != 0ull;
+                if (lane == 0) {
+                    st.iters++;
+                    st.setup_iters += any;
+                }
+            }
             if (dir == 0) {
-                unsigned long long kk = 0;
-                if (glane == 0) kk = atomicAdd(&s_next, 1ull);
-                kk = (unsigned long long)__shfl((long long)kk, gleader, 64);
-                if ((long long)kk >= w.p1) {
+                if (qi == qn && more) {
+                    qn = refill_pairs<kG, kStats>(A, w, gitem, L, los, &s_next, q, lane, more, st);
+                    qi = 0;
+                }
+                if (qi < qn) {
+                    activate_pair<kStats>(q[qi], P, glane, bintov);
+                    k = q[qi].k;
+                    ++qi;
+                    dir = 1;
+                    z0 = P.zmax;
+                    c = 0;
+                } else if (!more) {
                     dir = 2;
-                } else if (setup_pair<kG>(A, w.line, L, los, A.pair_part[kk], P, glane, kG * kRoundsMax)) {
-                    k = (long long)kk;
-                    if (kStats) st.pairs += (glane == 0);
-                    if (P.fp64) {
-                        if (kStats) st.pairs64 += (glane == 0);
-                        if (glane == 0) push_fixup(A, k, gitem, P.zmax, half, 0);
-                    } else {
-                        dir = 1;
-                        z0 = P.zmax;
-                        c = 0;
-                    }
                 }
             }
             const bool walking = (dir == 1 || dir == -1);
             if (!__ballot(dir != 2)) break;
+            long long tc1 = 0;
+            if (kStats) tc1 = clock64();
             if (walking && !P.wing_ready && step_needs_wing(P, bintov, z0, dir, c, 1, half)) {
                 wing_setup<kG>(P, bintov, glane);
                 if (glane == 0) P.wing_ready = 1;
                 wave_sync();
             }
+            long long tc2 = 0;
+            if (kStats) tc2 = clock64();
             int walked = 0, adv = 0;
             const bool done = walk_step<kG, kLds, kStats>(A, P, sbuf, row, gtab, z0, dir, half, c, lane, st, k, gitem,
                                                           walking, walked, adv);
+            if (kStats && lane == 0) {
+                const long long tc3 = clock64();
+                st.cyc_setup += tc1 - tc0;
+                st.cyc_wing += tc2 - tc1;
+                st.cyc_walk += tc3 - tc2;
+            }
             if (walking) {
                 if (done && walked >= 0 && glane == 0) record_window(A, w.line, k, dir, walked);
                 if (!done) {
@@ -847,6 +1037,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         atomicAdd((unsigned long long *)&A.stats[5], (unsigned long long)st.evaluated);
         atomicAdd((unsigned long long *)&A.stats[6], (unsigned long long)st.sums);
         atomicAdd((unsigned long long *)&A.stats[7], (unsigned long long)st.far);
+        if (lane == 0) {
+            atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)st.cyc_setup);
+            atomicAdd((unsigned long long *)&A.stats[9], (unsigned long long)st.cyc_wing);
+            atomicAdd((unsigned long long *)&A.stats[10], (unsigned long long)st.cyc_walk);
+            atomicAdd((unsigned long long *)&A.stats[11], (unsigned long long)st.iters);
+            atomicAdd((unsigned long long *)&A.stats[12], (unsigned long long)st.setup_iters);
+        }
     }
 }
 

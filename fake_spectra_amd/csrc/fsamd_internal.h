@@ -111,6 +111,7 @@ struct DepositArgs {
     const LineDesc *lines;      // device [nlines], lines of this launch
     int nlines;
     int nitems;
+    int claim;                  // pairs a lane group claims at once (tau kernel; 1..8)
     int *queue;                 // work-queue counter (zeroed before launch)
     int *win;                   // optional [nlines][npairs][2] pixels walked per direction
     long long npairs;

@@ -69,12 +69,12 @@ __device__ inline VoigtA make_voigt_a(double a)
     VoigtA v;
     v.a = (float)a;
     v.a2 = (float)(a * a);
-    v.a2_6 = (float)(a * a / 6.0);
+    v.a2_6 = (float)(a * a * (1.0 / 6.0));
     // exp(a^2) for a <= kA32Max: the series to a^4 is exact in FP32 (a^6/6 < 1e-9)
     v.ea2 = (float)(1.0 + a * a * (1.0 + 0.5 * a * a));
     v.two_a_sqpi = (float)(a * kTwoOverSqrtPi);
-    v.a_sqpi = (float)(a / kSqrtPi);
-    v.a3_sqpi = (float)(a * a * a / kSqrtPi);
+    v.a_sqpi = (float)(a * (1.0 / kSqrtPi));
+    v.a3_sqpi = (float)(a * a * a * (1.0 / kSqrtPi));
     v.two_a = (float)(2.0 * a);
     v.a2_log2e = a * a * kLog2E;
     return v;

@@ -264,10 +264,12 @@ def colden(plan: Plan, nbins: int, kernel: int, velfac: float, dens: Sequence[to
 
 
 def last_stats():
-    buf = (C.c_longlong * 8)()
-    N.check(N.lib().fsamd_last_stats(buf, 8))
+    buf = (C.c_longlong * 13)()
+    N.check(N.lib().fsamd_last_stats(buf, 13))
     return {"pairs": buf[0], "pixels": buf[1], "fp64_pixels": buf[2], "fp64_pairs": buf[3],
             "voigt_samples": buf[4], "evaluated_pixels": buf[5], "kernel_sums": buf[6], "far_pixels": buf[7],
+            "cycles_setup": buf[8], "cycles_wing": buf[9], "cycles_walk": buf[10], "wave_iterations": buf[11],
+            "setup_iterations": buf[12],
             "gpu_ms": float(N.lib().fsamd_last_ms()), "kernel_ms": float(N.lib().fsamd_last_kernel_ms())}
 
 

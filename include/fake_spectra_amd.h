@@ -120,13 +120,15 @@ int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double bo
  * the tail cut, pairs evaluated entirely in FP64, Voigt samples of the
  * deposited pixels (7 x sub-samples, as the reference counts them), pixels
  * evaluated (deposited or not), 7-sample kernel sums evaluated (one per
- * pixel sub-sample), pixels evaluated by the far-wing expansion; fsamd_last_stats
- * copies the first min(n, FSAMD_NSTATS).  And the GPU time of its launches in ms (hipEvents on
+ * pixel sub-sample), pixels evaluated by the far-wing expansion, then the
+ * deposit kernel's wavefront cycles (s_memtime) in pair set-up, in far-wing
+ * set-up and in walk steps, its wavefront loop iterations and the iterations
+ * that set up a pair; fsamd_last_stats copies the first min(n, FSAMD_NSTATS).  And the GPU time of its launches in ms (hipEvents on
  * the call's stream).  fsamd_set_stats(flags): bit 0 collects the counters
  * and the time, bit 1 the time only (the kernels then run without their
  * counting code); either makes fsamd_tau synchronize the stream. */
 int fsamd_set_stats(int flags);
-#define FSAMD_NSTATS 8
+#define FSAMD_NSTATS 13
 int fsamd_last_stats(long long *out, int n);
 double fsamd_last_ms(void);
 /* GPU time of the deposit kernel launches alone (the tau or colden kernel,

@@ -195,6 +195,17 @@ def test_cold_gas_uses_fp64_profile(oracle):
     assert_tau_close(got, oracle.interpolate(*args), "cold gas")
 
 
+@pytest.mark.parametrize("temps", [(3000.0, 20000.0), (1500.0, 3000.0, 20000.0), (1500.0, 1800.0)])
+def test_profile_regimes_match_oracle(oracle, temps):
+    """H I pairs at damping parameters from 4e-4 to 1.2e-3 (20000 K down
+    to 1500 K), one value per sightline set and mixed within wavefronts."""
+    w = synth.config(0, nlos=8, n_side=12)
+    for i, t in enumerate(temps):
+        w.temp[i::len(temps)] = np.float32(t)
+    args = w.interp_args(HI_1215, True)
+    assert_tau_close(SP._Particle_Interpolate(*args), oracle.interpolate(*args), "temps %s" % (temps,))
+
+
 def test_stats_counters_consistent():
     """The counting kernels deposit the same spectra as the plain ones (up to
     the order of the FP64 additions), and
