@@ -1119,49 +1119,22 @@ __device__ inline void adj_inner32(const PairTau &P, double vouter, const float 
     }
 }
 
-__device__ inline void adj_pixel(const PairTau &P, double vlow, double bintov, const float *gtab, double &S,
-                                 double &SU, double &SF)
+// Far-wing pixel (|U| >= ufar) of the adjoint: S, SU and SF by the moment
+// expansion of the forward and its derivative.
+__device__ inline void adj_far(const PairTau &P, double vlow, double bintov, double &S, double &SU, double &SF)
 {
     const double U = (vlow + 0.5 * bintov) * P.inv_b;
-    if (fabs(U) >= P.ufar) {
-        const float Y = P.ufar2 * __builtin_amdgcn_rcpf((float)(U * U));
-        float s = P.Cw[kWingTerms - 1], d = kWingTerms * P.Cw[kWingTerms - 1], f = P.CwF[kWingTerms - 1];
+    const float Y = P.ufar2 * __builtin_amdgcn_rcpf((float)(U * U));
+    float s = P.Cw[kWingTerms - 1], d = kWingTerms * P.Cw[kWingTerms - 1], f = P.CwF[kWingTerms - 1];
 #pragma unroll
-        for (int t = kWingTerms - 2; t >= 0; --t) {
-            s = __builtin_fmaf(s, Y, P.Cw[t]);
-            d = __builtin_fmaf(d, Y, (t + 1.0f) * P.Cw[t]);
-            f = __builtin_fmaf(f, Y, P.CwF[t]);
-        }
-        S = (double)(s * Y);
-        SU = 2.0 / U * (double)(d * Y);
-        SF = (double)(f * Y);
-        return;
+    for (int t = kWingTerms - 2; t >= 0; --t) {
+        s = __builtin_fmaf(s, Y, P.Cw[t]);
+        d = __builtin_fmaf(d, Y, (t + 1.0f) * P.Cw[t]);
+        f = __builtin_fmaf(f, Y, P.CwF[t]);
     }
-    const double vhighp = vlow + bintov;
-    const double width = vhighp - vlow;
-    float s = 0, su = 0, sf = 0;
-    if (width < P.b / 2.) {
-        adj_inner32(P, (vhighp + vlow) / 2., gtab, s, su, sf);
-        S = s;
-        SU = su;
-        SF = sf;
-        return;
-    }
-    const int np = 2 * ceil(width / (P.b / 2.) / 2) + 1.;
-    const double dv = width / (np - 1);
-    double ts = 0, tu = 0, tf = 0;
-    for (int i = 0; i < np; ++i) {
-        const double vv = (i == np - 1) ? vhighp : i * dv + vlow;
-        const double wt = (i == 0 || i == np - 1) ? 0.5 : 1.0;
-        s = su = sf = 0;
-        adj_inner32(P, vv, gtab, s, su, sf);
-        ts += wt * s;
-        tu += wt * su;
-        tf += wt * sf;
-    }
-    S = ts / (np - 1);
-    SU = tu / (np - 1);
-    SF = tf / (np - 1);
+    S = (double)(s * Y);
+    SU = 2.0 / U * (double)(d * Y);
+    SF = (double)(f * Y);
 }
 
 template <int kG>
@@ -1221,28 +1194,68 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             const long long wi = ((long long)w.line * A.npairs + k) * 2;
             const int lf = A.win[wi], lb = A.win[wi + 1];
             const int n = lf + lb;
-            const int zmax = P.zmax;
+            const int zlo = P.zmax - lb;   // the window: pixels zlo .. zlo + n - 1
             double aS = 0, aU = 0, aF = 0;
+            // far-wing pixels (|U| >= ufar) sit at the two ends of the window
+            int nfl = 0, nfr = 0;
             for (int c = 0; c < n; c += kG) {
-                const int idx = c + glane;
-                const bool valid = idx < n;
-                const int z = idx < lf ? zmax + idx : zmax - 1 - (idx - lf);
-                const double vlow = z * bintov - P.vel;
+                const int j = c + glane;
+                const double U = ((zlo + j) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
+                const bool far = j < n && fabs(U) >= P.ufar;
+                nfl += __popcll(group_bits<kG>(__ballot(far && U < 0), lane));
+                nfr += __popcll(group_bits<kG>(__ballot(far && U > 0), lane));
+            }
+            if (nfl + nfr > 0) {
                 if (!P.wing_ready) {
-                    const double U = (vlow + 0.5 * bintov) * P.inv_b;
-                    if (group_bits<kG>(__ballot(valid && fabs(U) >= P.ufar), lane)) {
-                        wing_setup<kG, true>(P, bintov, glane);
-                        if (glane == 0) P.wing_ready = 1;
-                        wave_sync();
+                    wing_setup<kG, true>(P, bintov, glane);
+                    if (glane == 0) P.wing_ready = 1;
+                    wave_sync();
+                }
+                for (int c = 0; c < nfl + nfr; c += kG) {
+                    const int f = c + glane;
+                    if (f < nfl + nfr) {
+                        const int z = zlo + (f < nfl ? f : n - nfr + (f - nfl));
+                        double S, SU, SF;
+                        adj_far(P, z * bintov - P.vel, bintov, S, SU, SF);
+                        const double g = grow[wrap_pixel(z, A.nbins)];
+                        aS += g * S;
+                        aU += g * SU;
+                        aF += g * SF;
                     }
                 }
-                if (valid) {
-                    double S, SU, SF;
-                    adj_pixel(P, vlow, bintov, gtab, S, SU, SF);
-                    const double g = grow[wrap_pixel(z, A.nbins)];
-                    aS += g * S;
-                    aU += g * SU;
-                    aF += g * SF;
+            }
+            // the other pixels on the shared sub-sample grid of tau_kern_outer
+            // (singleabs.h:110-125): each grid point once, weighted by the
+            // trapezoid weights and dL/dtau of the pixels it belongs to
+            const int j0 = nfl, nn = n - nfl - nfr;
+            if (nn > 0) {
+                const int nm1 = P.np - 1;
+                const int npts = nm1 == 0 ? nn : nn * nm1 + 1;
+                for (int m = glane; m < npts; m += kG) {
+                    double v, wgt;
+                    if (nm1 == 0) {
+                        const int z = zlo + j0 + m;
+                        const double vlow = z * bintov - P.vel;   // absorption.cpp:252
+                        v = ((vlow + bintov) + vlow) / 2.;        // singleabs.h:113
+                        wgt = grow[wrap_pixel(z, A.nbins)];
+                    } else {
+                        const int jj = m / nm1, kk = m - jj * nm1;
+                        const int z = zlo + j0 + jj;
+                        const double vlow = z * bintov - P.vel;
+                        v = kk == 0 ? vlow : kk * P.dv + vlow;    // singleabs.h:118-121
+                        if (kk == 0) {
+                            const double gl = jj > 0 ? grow[wrap_pixel(z - 1, A.nbins)] : 0.0;
+                            const double gr = jj < nn ? grow[wrap_pixel(z, A.nbins)] : 0.0;
+                            wgt = 0.5 * (gl + gr) * P.inv_nm1;
+                        } else {
+                            wgt = grow[wrap_pixel(z, A.nbins)] * P.inv_nm1;
+                        }
+                    }
+                    float s = 0, su = 0, sf = 0;
+                    adj_inner32(P, v, gtab, s, su, sf);
+                    aS += wgt * s;
+                    aU += wgt * su;
+                    aF += wgt * sf;
                 }
             }
             aS = group_sum<kG>(aS);
