@@ -177,7 +177,6 @@ struct PairTau {
     int wing_ready;  // Cw computed
     double dv;       // sub-sample spacing bintov / (np - 1)
     double inv_nm1;  // 1 / (np - 1)
-    float inv_nm1f;
     int nm1, rounds, pstep;   // near-step shape of the main kernel (step_shape)
 };
 
@@ -517,7 +516,6 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
         P.wing_ready = 0;
         P.inv_nm1 = C.np > 1 ? 1.0 / (C.np - 1) : 1.0;
         P.dv = C.np > 1 ? bintov * P.inv_nm1 : 0.0;
-        P.inv_nm1f = (float)P.inv_nm1;
         const StepShape sh = step_shape(C.np, kG);
         P.nm1 = sh.nm1;
         P.rounds = sh.rounds;
@@ -627,7 +625,6 @@ __device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane, d
         P.wing_ready = 0;
         P.inv_nm1 = r.inv_nm1;
         P.dv = nm1 > 0 ? bintov * r.inv_nm1 : 0.0;
-        P.inv_nm1f = (float)r.inv_nm1;
         P.nm1 = nm1;
         P.rounds = (r.shape >> 8) & 0xff;
         P.pstep = r.shape >> 16;
@@ -785,15 +782,14 @@ __device__ inline bool settle(const DepositArgs &A, const PairTau &P, double *ro
 // pixels the step covered.
 template <int kG, bool kLds, bool kStats>
 __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *sbuf, double *row,
-                                 const float *gtab, int z0, int dir, int count, int c, int lane, LaneStats &st,
-                                 long long k, int item, bool active, int &walked, int &adv)
+                                 const float *gtab, int z0, int dir, int count, int c, bool far, int lane,
+                                 LaneStats &st, long long k, int item, bool active, int &walked, int &adv)
 {
     adv = 0;
     if (!active) return false;
     const int glane = lane % kG;
     const double bintov = A.vbox / A.nbins;
-    const double U0 = ((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b;
-    if (fabs(U0) >= P.ufar) {
+    if (far) {
         for (int r = 0; r < kFarRounds; ++r) {
             const int cc = c + r * kG;
             const int pc = min(kG, count - cc);
@@ -826,6 +822,7 @@ __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *
     {
         const int nsub = nm1 == 0 ? pcount : pcount * nm1 + 1;
         const int zlo = dir > 0 ? z0 + c : z0 - c - (pcount - 1);
+        const double vlo = zlo * bintov - P.vel;   // vlow of the step's lowest pixel (absorption.cpp:252)
         for (int r = 0; r < P.rounds; ++r) {
             const int m = glane + r * kG;
             if (m < nsub) {
@@ -834,11 +831,10 @@ __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *
                     const double vlow = (z0 + dir * (c + m)) * bintov - P.vel;   // absorption.cpp:252
                     v = ((vlow + bintov) + vlow) / 2.;                           // singleabs.h:113
                 } else {
-                    // sub-sample kk of pixel zlo + j, in increasing velocity
-                    const int j = (int)(((float)m + 0.5f) * P.inv_nm1f);
-                    const int kk = m - j * nm1;
-                    const double vlow = (zlo + j) * bintov - P.vel;
-                    v = kk == 0 ? vlow : kk * P.dv + vlow;                       // singleabs.h:118-121
+                    // sub-sample m of the step, in increasing velocity: sub-sample
+                    // m mod (np-1) of its pixel (singleabs.h:118-121), up to the
+                    // rounding of m dv against the pixel's own vlow
+                    v = __builtin_fma((double)m, P.dv, vlo);
                 }
                 sbuf[m] = (float)inner32(P, v, gtab);
                 if (kStats) st.sums++;
@@ -988,7 +984,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if (!__ballot(dir != 2)) break;
             long long tc1 = 0;
             if (kStats) tc1 = clock64();
-            if (walking && !P.wing_ready && step_needs_wing(P, bintov, z0, dir, c, 1, half)) {
+            // the step's first pixel in the far wing: |U| only grows along a walk
+            const bool far = walking && fabs(((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b) >= P.ufar;
+            if (far && !P.wing_ready) {
                 wing_setup<kG>(P, bintov, glane);
                 if (glane == 0) P.wing_ready = 1;
                 wave_sync();
@@ -996,8 +994,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             long long tc2 = 0;
             if (kStats) tc2 = clock64();
             int walked = 0, adv = 0;
-            const bool done = walk_step<kG, kLds, kStats>(A, P, sbuf, row, gtab, z0, dir, half, c, lane, st, k, gitem,
-                                                          walking, walked, adv);
+            const bool done = walk_step<kG, kLds, kStats>(A, P, sbuf, row, gtab, z0, dir, half, c, far, lane, st, k,
+                                                          gitem, walking, walked, adv);
             if (kStats && lane == 0) {
                 const long long tc3 = clock64();
                 st.cyc_setup += tc1 - tc0;
