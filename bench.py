@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-near-lines", action="store_true")
     ap.add_argument("--adjoint-config", type=int, default=3,
                     help="workload of the forward vs forward+adjoint leg (3: 128^3, 1024 x 1024, H I 1215)")
     ap.add_argument("--colden", action="store_true",
@@ -102,6 +103,69 @@ def cpu_baseline(w, budget_s):
                       "index table + deposit, %.1f s" % (
                           "reference C++ -O2 -ffast-math OpenMP" if kind == "reference" else "C restatement",
                           "+".join(ln.name for ln in w.lines), len(sel), w.nlos, k, w.npart, t)}
+
+
+def near_lines_leg(w, pos, hh, ax, cofm, with_cpu, budget_s):
+    """_near_lines (py_module.cpp:25-99), the prefilter timed on its own: the
+    GPU on the full, device-resident particle set (binning + compaction, median
+    of 3 calls) and, on the host, the reference on every k-th particle with all
+    sightlines (the sample grown to about budget_s), both per particle."""
+    import numpy as np
+    import torch
+    from fake_spectra_amd import spectra_priv as SP
+
+    def run():
+        plan = SP.Plan(w.box, pos, hh, ax, cofm)
+        ids = plan.near_lines()
+        plan.close()
+        return ids
+
+    ids = run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ids = run()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    gpu_s = sorted(ts)[1]
+    out = {"unit": "particles/s", "particles": w.npart, "near_particles": int(ids.numel()),
+           "gpu_ms": 1e3 * gpu_s, "gpu_value": w.npart / gpu_s}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        kind = "reference" if O.ref_available() else "port"
+        lib = O.load_ref() if kind == "reference" else O.load_oracle()
+        n = min(w.npart, 200000)
+        while True:
+            k = max(1, w.npart // n)
+            t0 = time.perf_counter()
+            lib.near_lines(w.box, w.pos[::k], w.hh[::k], w.axis, w.cofm)
+            t = time.perf_counter() - t0
+            m = len(range(0, w.npart, k))
+            if t >= budget_s / 3 or k == 1:
+                break
+            n = min(w.npart, max(2 * n, int(n * budget_s / 3 / max(t, 1e-3))))
+        out["cpu_value"] = m / t
+        out["cpu_kind"] = kind
+        out["cpu_sample"] = "every %d-th particle (%d), all %d sightlines, %.1f s, %d threads" % (
+            k, m, w.nlos, t, lib.max_threads())
+        out["gpu_over_cpu"] = out["gpu_value"] / out["cpu_value"]
+    return out
+
+
+def host_cpu():
+    """The host's CPU model and logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
 
 
 def adjoint_leg(steps, config=3, w=None):
@@ -186,7 +250,8 @@ def cyc_split(st):
     if tot <= 0 or st["wave_iterations"] <= 0:
         return None
     return {"setup": st["cycles_setup"] / tot, "wing": st["cycles_wing"] / tot, "walk": st["cycles_walk"] / tot,
-            "iterations_with_setup": st["setup_iterations"] / st["wave_iterations"]}
+            "iterations_with_setup": st["setup_iterations"] / st["wave_iterations"],
+            "wave_iterations": st["wave_iterations"]}
 
 
 def main():
@@ -348,6 +413,9 @@ def main():
                         "note": "deposit only, same binning; one launch per sightline and line"}
         plan.close()
         progress("plain: %.2f s" % tplain)
+    if not args.no_near_lines:
+        out["near_lines"] = near_lines_leg(w, pos, hh, ax, cofm, not args.no_cpu, args.cpu_seconds / 3)
+        progress("near_lines done")
     if args.colden:
         out["colden"] = colden_leg(w, pos, hh, ax, cofm, dens[0], args.steps)
         progress("colden done")
@@ -357,6 +425,7 @@ def main():
         progress("adjoint leg done")
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_seconds)
+        cb["host"] = host_cpu()
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
     print(json.dumps(out))
