@@ -269,9 +269,9 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         if (tau && (rc = launch_pack_records(pos, vel, temp, h, A.lines, K, P->npart, P->rec.as<PartRec>(), s)))
             return rc;
         A.queue = P->queue.as<int>() + g;
-        // whole-sightline items: a lane group sets up 8 pairs at a time; the
-        // one-launch-per-sightline mode, whose items are short, claims one
-        A.claim = plain ? 1 : 8;
+        // whole-sightline items: a lane group sets up kPairQueue pairs at a
+        // time; the one-launch-per-sightline mode, whose items are short, one
+        A.claim = plain ? 1 : kPairQueue;
         A.stats = g_stats_on ? P->stats.as<long long>() : nullptr;
         A.npairs = P->npairs;
         A.win = rec ? P->win.as<int>() + (size_t)k0 * 2 * P->npairs : nullptr;

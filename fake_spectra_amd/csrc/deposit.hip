@@ -529,7 +529,7 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
 // queue of compact records, so that the FP64 set-up arithmetic runs on kQ
 // lanes at once; activating a queued pair only expands its record into the
 // group's PairTau slot.
-constexpr int kQ = 8;
+constexpr int kQ = kPairQueue;
 
 struct PairLite {
     double vel, inv_b, scale, vhigh, aa, inv_nm1;
@@ -871,8 +871,8 @@ __device__ inline void record_window(const DepositArgs &A, int line, long long k
     if (A.win) A.win[((long long)line * A.npairs + k) * 2 + (dir < 0 ? 1 : 0)] = len;
 }
 
-// Whether the next step of a walk reaches pixels that use the far-wing
-// expansion (|U| only grows along a walk after its first pixel).
+// Whether the next kg pixels of a fix-up walk reach pixels that use the
+// far-wing expansion (|U| only grows along a walk after its first pixel).
 __device__ inline bool step_needs_wing(const PairTau &P, double bintov, int z0, int dir, int c, int kg, int count)
 {
     const int last = min(c + kg - 1, count - 1);
