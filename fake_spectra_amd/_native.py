@@ -26,7 +26,8 @@ EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan
            "fsamd_colden", "fsamd_particle_interpolate", "fsamd_set_stats", "fsamd_last_stats", "fsamd_last_ms",
            "fsamd_voigt_peak", "fsamd_set_group_size", "fsamd_plan_keep_windows", "fsamd_tau_adjoint",
            "fsamd_tau_autotune", "fsamd_plan_overflow", "fsamd_mean_flux_scale", "fsamd_res_corr",
-           "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms", "fsamd_set_block_waves")
+           "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms", "fsamd_set_block_waves",
+           "fsamd_voigt_eval")
 
 
 class FsamdLine(C.Structure):
@@ -77,6 +78,8 @@ def _declare(L):
     L.fsamd_last_kernel_ms.restype = d
     L.fsamd_voigt_peak.argtypes = [d, d, C.POINTER(d), vp]
     L.fsamd_voigt_peak.restype = i
+    L.fsamd_voigt_eval.argtypes = [vp, ll, d, vp, vp]
+    L.fsamd_voigt_eval.restype = i
     L.fsamd_set_group_size.argtypes = [i]
     L.fsamd_set_group_size.restype = i
     L.fsamd_set_block_waves.argtypes = [i]

@@ -2,6 +2,7 @@
 #include "fsamd_internal.h"
 #include "plan.h"
 #include "deposit.h"
+#include "voigt.h"
 
 #include <algorithm>
 #include <cmath>
@@ -623,6 +624,14 @@ int fsamd_set_block_waves(int w)
     if (w != 2 && w != 4 && w != 8) return fail(FSAMD_ERR_ARG, "wavefronts per block must be 2, 4 or 8");
     g_block_waves = w;
     return FSAMD_OK;
+}
+
+int fsamd_voigt_eval(const double *u, long long n, double a, float *out, void *stream)
+{
+    if (n < 0 || (n > 0 && (!u || !out))) return fail(FSAMD_ERR_ARG, "bad argument");
+    if (!(a >= 0.0 && a <= (double)kA32Max))
+        return fail(FSAMD_ERR_ARG, "the FP32 profile takes 0 <= a <= 0.04 (larger a take the FP64 path)");
+    return launch_voigt_eval(u, n, a, out, (hipStream_t)stream);
 }
 
 int fsamd_voigt_peak(double umax, double a, double *samples_per_s, void *stream)

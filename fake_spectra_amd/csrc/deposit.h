@@ -24,6 +24,7 @@ int effective_waves(bool lds, bool stats, int waves);
 int launch_tau_adjoint(int gsize, const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);
 int adjoint_blocks_per_cu(int gsize, int nbins);
 int launch_tau_fixup(const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);
+int launch_voigt_eval(const double *u, long long n, double a, float *out, hipStream_t s);
 int launch_voigt_peak(long long per_thread, int nblocks, double du, double umax, float a, float *sink,
                       hipStream_t s);
 

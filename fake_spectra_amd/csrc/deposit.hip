@@ -1411,6 +1411,27 @@ __global__ __launch_bounds__(kThreads) void voigt_peak_kernel(long long per_thre
     if (acc == -1.0f) sink[0] = acc;   // keeps the loop live; never true
 }
 
+// H(a, u_i) by the deposit kernel's FP32 profile, one lane per point (the
+// profile's accuracy check); the damping parameter is set up as for a pair.
+__global__ __launch_bounds__(kThreads) void voigt_eval_kernel(const double *u, long long n, double a, float *out)
+{
+    __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
+    for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
+    __syncthreads();
+    const VoigtA va = make_voigt_a(a);
+    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
+        out[i] = voigt32(va, u[i], gtab);
+}
+
+int launch_voigt_eval(const double *u, long long n, double a, float *out, hipStream_t s)
+{
+    if (n <= 0) return FSAMD_OK;
+    const long long nb = std::min<long long>((n + kThreads - 1) / kThreads, 4096);
+    voigt_eval_kernel<<<(int)nb, kThreads, 0, s>>>(u, n, a, out);
+    FSAMD_HIP(hipGetLastError());
+    return FSAMD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 

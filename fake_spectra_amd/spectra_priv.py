@@ -273,6 +273,17 @@ def last_stats():
             "gpu_ms": float(N.lib().fsamd_last_ms()), "kernel_ms": float(N.lib().fsamd_last_kernel_ms())}
 
 
+def voigt_profile(u, a: float) -> torch.Tensor:
+    """H(a, u) = Re w(u + ia) by the deposit kernel's FP32 profile at the
+    points u (a CUDA tensor or array, evaluated in FP64 offsets), 0 <= a <= 0.04."""
+    dev = device()
+    uu = torch.as_tensor(np.asarray(u, dtype=np.float64) if not isinstance(u, torch.Tensor) else u,
+                         dtype=torch.float64).to(dev).contiguous().reshape(-1)
+    out = torch.empty(uu.numel(), dtype=torch.float32, device=dev)
+    N.check(N.lib().fsamd_voigt_eval(_ptr(uu), uu.numel(), float(a), _ptr(out), _stream()))
+    return out
+
+
 def set_stats(enable: bool, timing_only: bool = False):
     """Collect the counters and GPU time of each tau call (last_stats), or
     with timing_only just the time (the kernels then run without counting)."""

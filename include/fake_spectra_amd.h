@@ -165,6 +165,11 @@ int fsamd_set_group_size(int g);
  * counting variant runs with 4).  Process-wide; default 4. */
 int fsamd_set_block_waves(int w);
 
+/* H(a, u) = Re w(u + ia) at n points u (device pointer, FP64) by the deposit
+ * kernel's FP32 profile, into out (device, FP32); 0 <= a <= 0.04.  No
+ * counterpart in the reference's interface: it exposes the evaluator that
+ * replaces profile() (singleabs.h:56-61) for accuracy checks. */
+int fsamd_voigt_eval(const double *u, long long n, double a, float *out, void *stream);
 /* Diagnostic with no counterpart in the reference: isolated throughput of the
  * FP32 sample evaluator (kernel weight x Voigt profile) in samples/s, over
  * arguments u swept uniformly in [0, umax) with damping parameter a, no
