@@ -152,8 +152,16 @@ __device__ inline int wrap_pixel(int z, int nbins)
 // Each wavefront keeps the constants of its current pair in an LDS slot: the
 // values are uniform across the wave, and reading them with broadcast LDS
 // loads keeps them out of the vector registers of the per-pixel loop.
+// One interior kernel sample of a pair (singleabs.h:146-157): its offset
+// vv_i / b and weight K(q_i), read together in the sample loop.
+struct KernelSample {
+    double cv;
+    float w;
+    float pad;
+};
+
 struct PairTau {
-    double cv[kNGrid - 1];   // kernel sample velocities / b
+    KernelSample smp[kNGrid - 1];   // kernel sample velocities / b and weights
     double vel;      // velfac*ppos + pvel (absorption.cpp:234)
     double b;        // btherm (absorption.cpp:220)
     double inv_b;
@@ -167,7 +175,6 @@ struct PairTau {
     double scale_nd; // amp * dvk / velfac (scale without the density)
     double temp;     // particle temperature
     int part;        // particle index
-    float W[kNGrid - 1];     // kernel weights K(q_i)
     float Cw[kWingTerms];
     float CwF[kWingTerms];   // adjoint: far-wing coefficients of sum w (u dH/du + a dH/da)
     VoigtA va;
@@ -203,7 +210,7 @@ __device__ inline double inner32(const PairTau &P, double vouter, const float *g
     const VoigtA va = P.va;
     float acc = 0.0f;
 #pragma unroll 1
-    for (int i = 0; i < kNGrid - 1; ++i) acc = __builtin_fmaf(P.W[i], voigt32(va, P.cv[i] - s, gtab), acc);
+    for (int i = 0; i < kNGrid - 1; ++i) acc = __builtin_fmaf(P.smp[i].w, voigt32(va, P.smp[i].cv - s, gtab), acc);
     return (double)acc;
 }
 
@@ -308,17 +315,17 @@ __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
         }
     }
     // m'_n = sum_k C(2n,2k) X_k Y_(n-k), X_k by running powers
-    const double x4 = (P.cv[4] * inv_u) * (P.cv[4] * inv_u);
-    const double x5 = (P.cv[5] * inv_u) * (P.cv[5] * inv_u);
-    const double x6 = (P.cv[6] * inv_u) * (P.cv[6] * inv_u);
-    const double w4 = (double)P.W[4] + (double)P.W[2];
-    const double w5 = (double)P.W[5] + (double)P.W[1];
-    const double w6 = (double)P.W[6] + (double)P.W[0];
+    const double x4 = (P.smp[4].cv * inv_u) * (P.smp[4].cv * inv_u);
+    const double x5 = (P.smp[5].cv * inv_u) * (P.smp[5].cv * inv_u);
+    const double x6 = (P.smp[6].cv * inv_u) * (P.smp[6].cv * inv_u);
+    const double w4 = (double)P.smp[4].w + (double)P.smp[2].w;
+    const double w5 = (double)P.smp[5].w + (double)P.smp[1].w;
+    const double w6 = (double)P.smp[6].w + (double)P.smp[0].w;
     double p4 = 1.0, p5 = 1.0, p6 = 1.0, mom = 0.0;
 #pragma unroll 1
     for (int k = 0; k < kWingTerms; ++k) {
         double xk = w4 * p4 + w5 * p5 + w6 * p6;
-        if (k == 0) xk += (double)P.W[3];
+        if (k == 0) xk += (double)P.smp[3].w;
         p4 *= x4;
         p5 *= x5;
         p6 *= x6;
@@ -491,8 +498,8 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
     const double bintov = A.vbox / A.nbins;
     wave_sync();   // the previous pair's constants are no longer read
     if (lane < kNGrid - 1) {
-        P.cv[lane] = ((lane + 1) * C.dvk - C.vhigh) * C.inv_b;
-        P.W[lane] = sample_weight(A, C, lane);
+        P.smp[lane].cv = ((lane + 1) * C.dvk - C.vhigh) * C.inv_b;
+        P.smp[lane].w = sample_weight(A, C, lane);
     }
     if (lane == 0) {
         P.vel = C.vel;
@@ -604,8 +611,8 @@ __device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane, d
     wave_sync();   // the previous pair's constants are no longer read
     const double dvk = 2. * r.vhigh / kNGrid;
     if (glane < kNGrid - 1) {
-        P.cv[glane] = ((glane + 1) * dvk - r.vhigh) * r.inv_b;
-        P.W[glane] = r.W[glane < 3 ? 3 - glane : glane - 3];   // K(q) is even in the sample offset
+        P.smp[glane].cv = ((glane + 1) * dvk - r.vhigh) * r.inv_b;
+        P.smp[glane].w = r.W[glane < 3 ? 3 - glane : glane - 3];   // K(q) is even in the sample offset
     }
     if (glane == 0) {
         const int nm1 = r.shape & 0xff;
@@ -1109,8 +1116,8 @@ __device__ inline void adj_inner32(const PairTau &P, double vouter, const float 
     const VoigtA va = P.va;
 #pragma unroll 1
     for (int i = 0; i < kNGrid - 1; ++i) {
-        const VoigtGrad g = voigt32_grad(va, P.cv[i] - s, gtab);
-        const float w = P.W[i];
+        const VoigtGrad g = voigt32_grad(va, P.smp[i].cv - s, gtab);
+        const float w = P.smp[i].w;
         S = __builtin_fmaf(w, g.H, S);
         SU = __builtin_fmaf(w, g.Hu, SU);
         SF = __builtin_fmaf(w, g.F, SF);

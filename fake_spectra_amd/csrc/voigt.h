@@ -85,7 +85,7 @@ __device__ inline float g_of_u(float u, const float *tab)
 {
     int idx = (int)(u * (1.0f / FSAMD_G_WIDTH));
     idx = idx < FSAMD_G_NPIECE - 1 ? idx : FSAMD_G_NPIECE - 1;
-    const float t = __builtin_fmaf(u, 2.0f / FSAMD_G_WIDTH, -(2.0f * (float)idx + 1.0f));
+    const float t = __builtin_fmaf(u, 1.0f / FSAMD_G_WIDTH, -(float)idx);   // s in [0, 1) of the piece
     const float *c = tab + idx * FSAMD_G_STRIDE;
     float r = c[FSAMD_G_DEG];
 #pragma unroll
@@ -113,12 +113,12 @@ __device__ inline float voigt32(const VoigtA &v, double u64, const float *tab)
 {
     static_assert(FSAMD_G_DEG == 6 && FSAMD_G_STRIDE == 8, "voigt32 evaluates 7 coefficients per row");
     const float uh = (float)__builtin_fabs(u64);
-    // exp(a^2 - u^2) = 2^-y, y = (u^2 - a^2) log2(e) in FP64: 2^-frac(y) in FP32
-    // and the integer part by ldexp (0 once y passes the FP32 range)
+    // exp(a^2 - u^2) = 2^y, y = -(u^2 - a^2) log2(e) in FP64: 2^(y - rint(y))
+    // in FP32 and the integer part by ldexp (0 once y passes the FP32 range)
     const double u2 = u64 * u64;
-    const double y = __builtin_fmin(__builtin_fma(u2, kLog2E, -v.a2_log2e), 512.0);
+    const double y = __builtin_fmax(__builtin_fma(u2, -kLog2E, v.a2_log2e), -512.0);   // -(u^2 - a^2) log2(e)
     const double n = __builtin_rint(y);
-    const float E = __builtin_ldexpf(__builtin_amdgcn_exp2f(-(float)(y - n)), -(int)n);
+    const float E = __builtin_ldexpf(__builtin_amdgcn_exp2f((float)(y - n)), (int)n);
     // cos(2au) to (2au)^6: the next term, (2au)^8/40320, is below 2e-8 for
     // a <= 0.04 where exp(-u^2) counts (u <= 5); u^2 clamped to 144
     const float c2 = v.four_a2 * __builtin_fminf((float)u2, 144.0f);
@@ -131,11 +131,13 @@ __device__ inline float voigt32(const VoigtA &v, double u64, const float *tab)
     const float x = __builtin_amdgcn_rcpf(p);
     const bool core = uh < kVoigtCoreMax;
     const float q = uh * (1.0f / FSAMD_G_WIDTH);
-    const float t = __builtin_fmaf(__builtin_amdgcn_fractf(q), 2.0f, -1.0f);
-    const int idx = core ? (int)q : FSAMD_G_NPIECE;
-    const float var = core ? t : x;
-    const float4 lo = *(const float4 *)(tab + idx * FSAMD_G_STRIDE);
-    const float4 hi = *(const float4 *)(tab + idx * FSAMD_G_STRIDE + 4);
+    // piece q (variable: the fractional part of q) or the wing row (variable x)
+    constexpr int kRowBytes = FSAMD_G_STRIDE * (int)sizeof(float);
+    const int off = core ? (int)q * kRowBytes : FSAMD_G_NPIECE * kRowBytes;
+    const float var = core ? __builtin_amdgcn_fractf(q) : x;
+    const float *row = (const float *)((const char *)tab + off);
+    const float4 lo = *(const float4 *)row;
+    const float4 hi = *(const float4 *)(row + 4);
     float r = __builtin_fmaf(hi.z, var, hi.y);
     r = __builtin_fmaf(r, var, hi.x);
     r = __builtin_fmaf(r, var, lo.w);
