@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-near-lines", action="store_true")
+    ap.add_argument("--no-drop-in", action="store_true")
     ap.add_argument("--adjoint-config", type=int, default=3,
                     help="workload of the forward vs forward+adjoint leg (3: 128^3, 1024 x 1024, H I 1215)")
     ap.add_argument("--colden", action="store_true",
@@ -152,6 +153,24 @@ def near_lines_leg(w, pos, hh, ax, cofm, with_cpu, budget_s):
             k, m, w.nlos, t, lib.max_threads())
         out["gpu_over_cpu"] = out["gpu_value"] / out["cpu_value"]
     return out
+
+
+def drop_in_leg(w):
+    """The numpy-in, numpy-out _Particle_Interpolate call (the reference's
+    boundary, py_module.cpp:103-233), one per line as spectra.py:673 makes
+    them: host-to-device copies of the particle arrays, binning, deposit and
+    the copy of the result back to the host, all timed (median of 3)."""
+    from fake_spectra_amd import spectra_priv as SP
+    SP._Particle_Interpolate(*w.interp_args(w.lines[0], True))
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for ln in w.lines:
+            SP._Particle_Interpolate(*w.interp_args(ln, True))
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[1]
+    return {"value": len(w.lines) * w.nlos * w.nbins / t, "unit": "sightline-pixels/s", "ms_per_step": 1e3 * t,
+            "note": "host numpy in and out (PCIe included), one call per line"}
 
 
 def host_cpu():
@@ -413,6 +432,9 @@ def main():
                         "note": "deposit only, same binning; one launch per sightline and line"}
         plan.close()
         progress("plain: %.2f s" % tplain)
+    if not args.no_drop_in:
+        out["drop_in"] = drop_in_leg(w)
+        progress("drop-in calls done")
     if not args.no_near_lines:
         out["near_lines"] = near_lines_leg(w, pos, hh, ax, cofm, not args.no_cpu, args.cpu_seconds / 3)
         progress("near_lines done")
