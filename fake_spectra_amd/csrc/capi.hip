@@ -458,7 +458,7 @@ int fsamd_plan_overflow(fsamd_plan *plan, int *flag, void *stream)
 
 int fsamd_tau_autotune(fsamd_plan *P, int nbins, int kernel, double velfac, double tautail, const float *pos,
                        const float *vel, const float *temp, const float *h, const fsamd_line *lines, int nlines,
-                       void *stream, int *best_g, int *best_waves, float *ms5)
+                       void *stream, int *best_g, int *best_waves, float *ms7)
 {
     if (!P || !best_g || !best_waves) return fail(FSAMD_ERR_ARG, "NULL argument");
     hipStream_t s = (hipStream_t)stream;
@@ -471,15 +471,14 @@ int fsamd_tau_autotune(fsamd_plan *P, int nbins, int kernel, double velfac, doub
     hipEvent_t e0, e1;
     FSAMD_HIP(hipEventCreate(&e0));
     FSAMD_HIP(hipEventCreate(&e1));
-    const int waves_alt[2] = {2, 8};
     float best = 1e30f;
     int bg = 16, bw = 4;
     int rc = FSAMD_OK;
-    // one untimed run makes the allocations; then the three widths at 4
-    // wavefronts per block, then 2 and 8 wavefronts at the best width
-    for (int i = -1; i < 5 && rc == FSAMD_OK; ++i) {
-        const int g = i < 0 ? 16 : i < 3 ? widths[i] : bg;
-        const int w = i < 3 ? 4 : waves_alt[i - 3];
+    // one untimed run makes the allocations; then the three widths at 4 and
+    // at 8 wavefronts per block, then 2 wavefronts at the best width
+    for (int i = -1; i < 7 && rc == FSAMD_OK; ++i) {
+        const int g = i < 0 ? 16 : i < 6 ? widths[i % 3] : bg;
+        const int w = i < 3 ? 4 : i < 6 ? 8 : 2;
         FSAMD_HIP(hipEventRecord(e0, s));
         rc = run_deposit(true, P, nbins, kernel, velfac, tautail, pos, vel, temp, h, lines, nlines,
                          FSAMD_MODE_BATCHED, s, stride, g, w);
@@ -489,7 +488,7 @@ int fsamd_tau_autotune(fsamd_plan *P, int nbins, int kernel, double velfac, doub
         float ms = 0;
         FSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
         if (i < 0) continue;
-        if (ms5) ms5[i] = ms;
+        if (ms7) ms7[i] = ms;
         if (ms < best) {
             best = ms;
             bg = g;

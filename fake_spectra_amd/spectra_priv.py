@@ -206,13 +206,12 @@ def autotune(plan: Plan, nbins: int, kernel: int, velfac: float, vel: torch.Tens
     scratch = [torch.empty((plan.nlos, nbins), dtype=torch.float64, device=plan.pos.device) for _ in quads]
     arr = _line_structs(quads, dens, scratch)
     best, waves = C.c_int(), C.c_int()
-    ms = (C.c_float * 5)()
+    ms = (C.c_float * 7)()
     N.check(N.lib().fsamd_tau_autotune(plan.handle, int(nbins), int(kernel), float(velfac), float(tautail),
                                        _ptr(plan.pos), _ptr(vel), _ptr(temp), _ptr(plan.h), arr, len(quads), _stream(),
                                        C.byref(best), C.byref(waves), ms))
-    times = {(16, 4): ms[0], (32, 4): ms[1], (64, 4): ms[2]}
-    times[(best.value, 2)] = ms[3]    # the wavefront counts are timed at the best width
-    times[(best.value, 8)] = ms[4]
+    times = {(16, 4): ms[0], (32, 4): ms[1], (64, 4): ms[2], (16, 8): ms[3], (32, 8): ms[4], (64, 8): ms[5]}
+    times[(best.value, 2)] = ms[6]    # 2 wavefronts are timed at the best width
     return best.value, waves.value, times
 
 

@@ -138,14 +138,14 @@ double fsamd_last_kernel_ms(void);
 /* Autotuner (no counterpart in the reference): times the batched tau launch
  * on every k-th sightline of the plan's heaviest-first order (k = nlos /
  * 16384, all sightlines below 32768) for lane-group widths 16, 32 and 64 at 4
- * wavefronts per sightline work item, then 2 and 8 wavefronts at the best
+ * and at 8 wavefronts per sightline work item, then 2 wavefronts at the best
  * width, and keeps the fastest pair for later fsamd_tau calls with the same
  * nbins, line count and kernel.  Writes partial results into the lines'
- * outputs.  ms5 (optional, host) receives the five times in that order. */
+ * outputs.  ms7 (optional, host) receives the seven times in that order. */
 int fsamd_tau_autotune(fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail,
                        const float *pos, const float *vel, const float *temp, const float *h,
                        const fsamd_line *lines, int nlines, void *stream, int *best_g, int *best_waves,
-                       float *ms5);
+                       float *ms7);
 
 /* fsamd_tau inside a hipGraph capture (the stream reports capturing):
  * allowed in the batched mode without statistics after one uncaptured call

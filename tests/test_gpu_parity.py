@@ -153,7 +153,7 @@ def test_autotune_picks_a_width(oracle):
     dens = [dev(w.dens_for(ln)) for ln in w.lines]
     best, waves, ms = SP.autotune(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
     assert best in (16, 32, 64) and waves in (2, 4, 8) and all(v > 0 for v in ms.values())
-    assert len(ms) == 5
+    assert len(ms) == 7
     outs = SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens)
     for ln, o in zip(w.lines, outs):
         assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)), "tuned " + ln.name)
