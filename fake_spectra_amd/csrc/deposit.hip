@@ -890,7 +890,7 @@ __device__ inline bool step_needs_wing(const PairTau &P, double bintov, int z0, 
 
 // Walk one direction with the whole wave (fix-up kernel): pixels z0, z0+dir,
 // ... for at most `count` pixels, until and including the tail cut.
-template <int kMode>
+template <int kMode, bool kStats>
 __device__ inline int walk64(const DepositArgs &A, PairTau &P, double *row, const float *gtab, int z0, int dir,
                              int count, int lane, LaneStats &st, long long k, int item)
 {
@@ -902,7 +902,7 @@ __device__ inline int walk64(const DepositArgs &A, PairTau &P, double *row, cons
             if (lane == 0) P.wing_ready = 1;
             wave_sync();
         }
-        if (walk_chunk<64, kMode, false, true>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item, true, walked))
+        if (walk_chunk<64, kMode, false, kStats>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item, true, walked))
             return walked;
     }
     return walked;
@@ -1054,6 +1054,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 
 // Deferred walks, after the deposit kernel has written its rows: one wave per
 // fix-up entry, deposits straight into the output with FP64 atomics.
+template <bool kStats>
 __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, const WorkItem *items)
 {
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
@@ -1073,18 +1074,18 @@ __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, cons
         const LosInfo los = los_info(A, w.los);
         if (!setup_pair<64>(A, w.line, L, los, A.pair_part[f.k], P, lane)) continue;
         if (f.dir == 0) {
-            const int lf = walk64<kWalkFull64>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k, f.item);
-            const int lb = walk64<kWalkFull64>(A, P, out_row, gtab, P.zmax - 1, -1, half, lane, st, f.k, f.item);
+            const int lf = walk64<kWalkFull64, kStats>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k, f.item);
+            const int lb = walk64<kWalkFull64, kStats>(A, P, out_row, gtab, P.zmax - 1, -1, half, lane, st, f.k, f.item);
             if (lane == 0) {
                 record_window(A, w.line, f.k, 1, lf);
                 record_window(A, w.line, f.k, -1, lb);
             }
         } else {
-            const int l = walk64<kWalkGuarded>(A, P, out_row, gtab, f.z0, f.dir, f.count, lane, st, f.k, f.item);
+            const int l = walk64<kWalkGuarded, kStats>(A, P, out_row, gtab, f.z0, f.dir, f.count, lane, st, f.k, f.item);
             if (lane == 0) record_window(A, w.line, f.k, f.dir, (half - f.count) + l);
         }
     }
-    if (A.stats) {
+    if (kStats && A.stats) {
         atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.pixels);
         atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st.px64);
         atomicAdd((unsigned long long *)&A.stats[4], (unsigned long long)st.samples);
@@ -1521,7 +1522,11 @@ int launch_deposit(bool tau, bool lds, int gsize, int waves, const DepositArgs &
 
 int launch_tau_fixup(const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s)
 {
-    tau_fixup_kernel<<<nblocks, kThreads, 0, s>>>(A, items);
+    // the counting variant only when counters are collected
+    if (A.stats)
+        tau_fixup_kernel<true><<<nblocks, kThreads, 0, s>>>(A, items);
+    else
+        tau_fixup_kernel<false><<<nblocks, kThreads, 0, s>>>(A, items);
     FSAMD_HIP(hipGetLastError());
     return FSAMD_OK;
 }
