@@ -669,7 +669,7 @@ enum { kWalkMain = 0, kWalkGuarded = 1, kWalkFull64 = 2 };
 // One step of kG consecutive pixels of a walk (lane glane of the group), each
 // pixel evaluated on its own as tau_kern_outer does (singleabs.h:104-126,
 // sub-sample grid from the pixel's own vhigh - vlow): the fix-up kernel's
-// walk (walk64), in the guarded or the all-FP64 mode.  Returns true when the
+// walk (walk_fix), in the guarded or the all-FP64 mode.  Returns true when the
 // walk is over (tail cut reached, end of the window, or in kWalkMain mode the
 // decision deferred).
 template <int kG, int kMode, bool kLds, bool kStats>
@@ -888,25 +888,30 @@ __device__ inline bool step_needs_wing(const PairTau &P, double bintov, int z0, 
     return fmax(fabs(U0), fabs(U1)) >= P.ufar;
 }
 
-// Walk one direction with the whole wave (fix-up kernel): pixels z0, z0+dir,
-// ... for at most `count` pixels, until and including the tail cut.
-template <int kMode, bool kStats>
-__device__ inline int walk64(const DepositArgs &A, PairTau &P, double *row, const float *gtab, int z0, int dir,
-                             int count, int lane, LaneStats &st, long long k, int item)
+// Walk one direction with a kG-lane group (fix-up kernel): pixels z0,
+// z0+dir, ... for at most `count` pixels, until and including the tail cut.
+template <int kG, int kMode, bool kStats>
+__device__ inline int walk_fix(const DepositArgs &A, PairTau &P, double *row, const float *gtab, int z0, int dir,
+                               int count, int lane, LaneStats &st, long long k, int item)
 {
     const double bintov = A.vbox / A.nbins;
+    const int glane = lane % kG;
     int walked = 0;
-    for (int c = 0; c < count; c += 64) {
-        if (kMode != kWalkFull64 && !P.wing_ready && step_needs_wing(P, bintov, z0, dir, c, 64, count)) {
-            wing_setup<64>(P, bintov, lane);
-            if (lane == 0) P.wing_ready = 1;
+    for (int c = 0; c < count; c += kG) {
+        if (kMode != kWalkFull64 && !P.wing_ready && step_needs_wing(P, bintov, z0, dir, c, kG, count)) {
+            wing_setup<kG>(P, bintov, glane);
+            if (glane == 0) P.wing_ready = 1;
             wave_sync();
         }
-        if (walk_chunk<64, kMode, false, kStats>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item, true, walked))
+        if (walk_chunk<kG, kMode, false, kStats>(A, P, row, gtab, z0, dir, count, c, lane, st, k, item, true, walked))
             return walked;
     }
     return walked;
 }
+
+// Lanes per fix-up entry: most deferred walks end a few pixels past the
+// deciding pixel, so a wavefront works on four entries at once.
+constexpr int kFixG = 16;
 
 // 4 waves per SIMD: measured 8% faster at c1 than the 3 the register count
 // allows by default (a few dwords of spill in the pair setup).  kStats adds
@@ -1057,32 +1062,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 template <bool kStats>
 __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, const WorkItem *items)
 {
+    constexpr int kGroups = kThreads / kFixG;
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
-    __shared__ PairTau pslot[kWaves];
+    __shared__ PairTau pslot[kGroups];
     for (int i = threadIdx.x; i < kGTableSize; i += kThreads) gtab[i] = c_gtab[i];
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    PairTau &P = pslot[wave];
+    const int lane = threadIdx.x & 63, gid = threadIdx.x / kFixG, glane = lane % kFixG;
+    PairTau &P = pslot[gid];
     LaneStats st;
     const int n = min(A.fix_count[0], A.fix_cap);
     const int half = A.nbins / 2;
-    for (int e = blockIdx.x * kWaves + wave; e < n; e += gridDim.x * kWaves) {
+    for (int e = blockIdx.x * kGroups + gid; e < n; e += gridDim.x * kGroups) {
         const Fixup f = A.fix[e];
         const WorkItem w = items[f.item];
         const LineDesc L = line_of(A, w.line);
         double *out_row = L.out + (size_t)w.los * A.nbins;
         const LosInfo los = los_info(A, w.los);
-        if (!setup_pair<64>(A, w.line, L, los, A.pair_part[f.k], P, lane)) continue;
+        if (!setup_pair<kFixG>(A, w.line, L, los, A.pair_part[f.k], P, glane)) continue;
         if (f.dir == 0) {
-            const int lf = walk64<kWalkFull64, kStats>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k, f.item);
-            const int lb = walk64<kWalkFull64, kStats>(A, P, out_row, gtab, P.zmax - 1, -1, half, lane, st, f.k, f.item);
-            if (lane == 0) {
+            const int lf = walk_fix<kFixG, kWalkFull64, kStats>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k,
+                                                                  f.item);
+            const int lb = walk_fix<kFixG, kWalkFull64, kStats>(A, P, out_row, gtab, P.zmax - 1, -1, half, lane, st,
+                                                                  f.k, f.item);
+            if (glane == 0) {
                 record_window(A, w.line, f.k, 1, lf);
                 record_window(A, w.line, f.k, -1, lb);
             }
         } else {
-            const int l = walk64<kWalkGuarded, kStats>(A, P, out_row, gtab, f.z0, f.dir, f.count, lane, st, f.k, f.item);
-            if (lane == 0) record_window(A, w.line, f.k, f.dir, (half - f.count) + l);
+            const int l = walk_fix<kFixG, kWalkGuarded, kStats>(A, P, out_row, gtab, f.z0, f.dir, f.count, lane, st,
+                                                                  f.k, f.item);
+            if (glane == 0) record_window(A, w.line, f.k, f.dir, (half - f.count) + l);
         }
     }
     if (kStats && A.stats) {
