@@ -104,6 +104,19 @@ def test_tau_configs_vs_oracle(oracle, idx, scale, nlos):
     assert_colden_close(col.cpu().numpy(), oracle.interpolate(*w.interp_args(None, False)), "c%d colden" % idx)
 
 
+@pytest.mark.parametrize("kernel", [0, 3])
+def test_kernels_on_a_workload_match_oracle(oracle, kernel):
+    """The tophat and the quintic SPH kernel (singleabs.h:31-42) on the
+    c1-like field, both lines, in one batched launch."""
+    w = synth.config(1, scale=1 / 64, nlos=12)
+    w.kernel = kernel
+    plan = plan_for(w)
+    outs = SP.tau(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), w.lines,
+                  [dev(w.dens_for(ln)) for ln in w.lines], w.tautail)
+    for ln, o in zip(w.lines, outs):
+        assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)), "kernel %d %s" % (kernel, ln.name))
+
+
 @pytest.mark.parametrize("g,waves", [(16, 4), (32, 4), (64, 4), (16, 2), (32, 8)])
 def test_group_sizes_match_oracle(oracle, g, waves):
     """Every lane-group width and wavefront count of the tau kernel gives
