@@ -391,10 +391,18 @@ __device__ inline void visit_accepted(const BinArgs &A, const float *p, float hh
             lo2 = 0;
             hi2 = nc - 1;
         }
+        // a window narrower than the grid lies within (-nc, 2nc): one add or
+        // subtract wraps a cell index (the modulo stays for anything else)
+        auto wrap = [nc](long long c) {
+            if (c >= 0 && c < nc) return (int)c;
+            if (c < 0 && c >= -nc) return (int)(c + nc);
+            if (c >= nc && c < 2 * (long long)nc) return (int)(c - nc);
+            return (int)(((c % nc) + nc) % nc);
+        };
         for (long long c2 = lo2; c2 <= hi2; ++c2) {
-            int w2 = (int)(((c2 % nc) + nc) % nc);
+            const int w2 = wrap(c2);
             for (long long c1 = lo1; c1 <= hi1; ++c1) {
-                int w1 = (int)(((c1 % nc) + nc) % nc);
+                const int w1 = wrap(c1);
                 const int cell = A.grid.base[ai] + w2 * nc + w1;
                 const int e = A.grid.cell_start[cell + 1];
                 for (int k = A.grid.cell_start[cell]; k < e; ++k) {
