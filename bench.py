@@ -276,8 +276,8 @@ def cyc_split(st):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each sightline is independent; "
-              "see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (sightlines are independent, so the "
+              "sightline work queue would be split across GPUs). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
