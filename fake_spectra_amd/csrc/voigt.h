@@ -56,7 +56,6 @@ struct VoigtA {
     float a;            // damping parameter
     float a2;           // a^2
     float a2_6;         // a^2 / 6
-    float ea2;          // exp(a^2)
     float two_a_sqpi;   // 2a/sqrt(pi)
     float a_sqpi;       // a/sqrt(pi)
     float a3_sqpi3;     // a^3/(3 sqrt(pi))
@@ -70,41 +69,12 @@ __device__ inline VoigtA make_voigt_a(double a)
     v.a = (float)a;
     v.a2 = (float)(a * a);
     v.a2_6 = (float)(a * a * (1.0 / 6.0));
-    // exp(a^2) for a <= kA32Max: the series to a^4 is exact in FP32 (a^6/6 < 1e-9)
-    v.ea2 = (float)(1.0 + a * a * (1.0 + 0.5 * a * a));
     v.two_a_sqpi = (float)(a * kTwoOverSqrtPi);
     v.a_sqpi = (float)(a * (1.0 / kSqrtPi));
     v.a3_sqpi3 = (float)(a * a * a * (1.0 / (3.0 * kSqrtPi)));
     v.four_a2 = (float)(4.0 * a * a);
     v.a2_log2e = a * a * kLog2E;
     return v;
-}
-
-// G(u) = 1 - 2u D(u) for 0 <= u < 7 from the table (in LDS or constant memory).
-__device__ inline float g_of_u(float u, const float *tab)
-{
-    int idx = (int)(u * (1.0f / FSAMD_G_WIDTH));
-    idx = idx < FSAMD_G_NPIECE - 1 ? idx : FSAMD_G_NPIECE - 1;
-    const float t = __builtin_fmaf(u, 1.0f / FSAMD_G_WIDTH, -(float)idx);   // s in [0, 1) of the piece
-    const float *c = tab + idx * FSAMD_G_STRIDE;
-    float r = c[FSAMD_G_DEG];
-#pragma unroll
-    for (int k = FSAMD_G_DEG - 1; k >= 0; --k) r = __builtin_fmaf(r, t, c[k]);
-    return r;
-}
-
-// exp(-(p + d)) with p = fl(u^2) and d its double-float remainder, p <= 49.
-__device__ inline float exp_neg_df(float p, float d)
-{
-    const float L2E = 1.44269502f;            // fl(log2 e)
-    const float L2E_LO = 1.925963e-08f;       // log2 e - L2E
-    const float th = p * L2E;
-    float tl = __builtin_fmaf(p, L2E, -th);
-    tl = __builtin_fmaf(p, L2E_LO, tl);
-    tl = __builtin_fmaf(d, L2E, tl);
-    const float n = __builtin_rintf(th);
-    const float f = (th - n) + tl;            // th - n is exact
-    return __builtin_ldexpf(__builtin_amdgcn_exp2f(-f), -(int)n);
 }
 
 // H(a, u) with u given in FP64 (the sample's offset from the line centre in
@@ -191,10 +161,12 @@ __device__ inline VoigtGrad voigt32_grad(const VoigtA &v, double u64, const floa
         r.F = v.a_sqpi * x * f;
         return r;
     }
-    const float ul = (float)(au - (double)uh);
-    const float p = uh * uh;
-    const float dd = __builtin_fmaf(2.0f * uh, ul, __builtin_fmaf(uh, uh, -p));
-    const float E = exp_neg_df(p, dd) * v.ea2;          // exp(a^2 - u^2)
+    // exp(a^2 - u^2) as in voigt32: the exponent in FP64, 2^frac in FP32
+    const double u2 = au * au;
+    const double y = __builtin_fmax(__builtin_fma(u2, -kLog2E, v.a2_log2e), -512.0);
+    const double ny = __builtin_rint(y);
+    const float E = __builtin_ldexpf(__builtin_amdgcn_exp2f((float)(y - ny)), (int)ny);
+    const float p = (float)u2;
     const float c = 2.0f * v.a * uh;
     const float c2 = c * c;
     float cs = __builtin_fmaf(c2, -1.0f / 40320.0f, 1.0f / 720.0f);
@@ -204,7 +176,19 @@ __device__ inline VoigtGrad voigt32_grad(const VoigtA &v, double u64, const floa
     float sn = __builtin_fmaf(c2, -1.0f / 5040.0f, 1.0f / 120.0f);
     sn = __builtin_fmaf(c2, sn, -1.0f / 6.0f);
     sn = c * __builtin_fmaf(c2, sn, 1.0f);
-    const float G = g_of_u(uh, gtab);
+    // G from the piece's row (two 16-byte reads), in the fractional part of u/width
+    const float q = uh * (1.0f / FSAMD_G_WIDTH);
+    const int idx = min((int)q, FSAMD_G_NPIECE - 1);
+    const float var = q - (float)idx;
+    const float *row = gtab + idx * FSAMD_G_STRIDE;
+    const float4 lo = *(const float4 *)row;
+    const float4 hi = *(const float4 *)(row + 4);
+    float G = __builtin_fmaf(hi.z, var, hi.y);
+    G = __builtin_fmaf(G, var, hi.x);
+    G = __builtin_fmaf(G, var, lo.w);
+    G = __builtin_fmaf(G, var, lo.z);
+    G = __builtin_fmaf(G, var, lo.y);
+    G = __builtin_fmaf(G, var, lo.x);
     float D;
     if (uh < 0.25f) {
         // D(u) = u - 2u^3/3 + 4u^5/15 - 8u^7/105 + 16u^9/945
