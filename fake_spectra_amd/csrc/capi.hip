@@ -5,6 +5,7 @@
 #include "voigt.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -17,8 +18,8 @@ static thread_local int g_stats_on = 0;    // device counters (and timing)
 static thread_local int g_timing_on = 0;   // hipEvents around the launches
 static thread_local double g_last_ms = 0;          // whole call
 static thread_local double g_last_kernel_ms = 0;   // deposit kernel launches only
-static int g_group_size = 16;   // pixels per lane group of the tau kernel (16, 32 or 64)
-static int g_block_waves = 4;   // wavefronts per block (per sightline work item): 2, 4 or 8
+static std::atomic<int> g_group_size{16};   // pixels per lane group of the tau kernel (16, 32 or 64)
+static std::atomic<int> g_block_waves{4};   // wavefronts per block (per sightline work item): 2, 4 or 8
 
 void set_error(const std::string &msg) { g_error = msg; }
 
@@ -211,8 +212,8 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         g_last_kernel_ms = 0;
     }
     const bool tuned = P->tuned_g > 0 && P->tuned_key == tune_key(nbins, nlines, kernel);
-    const int gsize = force_g ? force_g : tuned ? P->tuned_g : g_group_size;
-    const int waves = force_waves ? force_waves : tuned ? P->tuned_waves : g_block_waves;
+    const int gsize = force_g ? force_g : tuned ? P->tuned_g : g_group_size.load(std::memory_order_relaxed);
+    const int waves = force_waves ? force_waves : tuned ? P->tuned_waves : g_block_waves.load(std::memory_order_relaxed);
     const long long chunk_key = plain ? kPlainChunk : (sample_stride > 1 ? -sample_stride : (1ll << 62));
     const bool rec = tau && P->keep_windows;
     if (rec) {
@@ -553,7 +554,7 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         FSAMD_HIP(hipStreamSynchronize(s));
         if ((rc = regrow(P, P->rec, rbytes))) return rc;
     }
-    const int gsize = g_group_size;
+    const int gsize = g_group_size.load(std::memory_order_relaxed);
     for (int g = 0; g < ngroups; ++g) {
         const int k0 = g * kMaxLines;
         const int K = std::min(kMaxLines, nlines - k0);

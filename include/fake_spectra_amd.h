@@ -4,6 +4,12 @@
  * All functions return FSAMD_OK (0) or an error code; fsamd_last_error()
  * gives the message of the last failure on the calling thread.
  *
+ * Threads: calls on different plans may run at the same time from different
+ * host threads.  Calls on one plan must not overlap: a plan holds the scratch
+ * of its calls (work items, queue counters, the fix-up list, packed particle
+ * records), so give each thread its own plan or serialise.  The tuning knobs
+ * (fsamd_set_group_size, fsamd_set_block_waves) are process-wide.
+ *
  * The entry points replace the reference's native path behind
  * Spectra.get_tau / Spectra.get_col_density:
  *   fsamd_plan_create        IndexTable ctor + get_near_particles
