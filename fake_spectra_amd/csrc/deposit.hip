@@ -209,7 +209,7 @@ __device__ inline double inner32(const PairTau &P, double vouter, const float *g
     const double s = vouter * P.inv_b;
     const VoigtA va = P.va;
     float acc = 0.0f;
-#pragma unroll 1
+#pragma unroll 2  // measured 0.7% faster at c1 than 1; a full unroll spills
     for (int i = 0; i < kNGrid - 1; ++i) acc = __builtin_fmaf(P.smp[i].w, voigt32(va, P.smp[i].cv - s, gtab), acc);
     return (double)acc;
 }
@@ -1124,7 +1124,7 @@ __device__ inline void adj_inner32(const PairTau &P, double vouter, const float 
 {
     const double s = vouter * P.inv_b;
     const VoigtA va = P.va;
-#pragma unroll 1
+#pragma unroll 2
     for (int i = 0; i < kNGrid - 1; ++i) {
         const VoigtGrad g = voigt32_grad(va, P.smp[i].cv - s, gtab);
         const float w = P.smp[i].w;
