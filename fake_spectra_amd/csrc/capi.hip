@@ -184,7 +184,20 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
     static thread_local DeviceInfo dev;
     int rc;
     if (dev.ncu == 0 && (rc = device_info(dev))) return rc;
-    const bool lds = deposit_lds_bytes(nbins) + 2048 <= (size_t)dev.max_lds;
+    const bool tuned = P->tuned_g > 0 && P->tuned_key == tune_key(nbins, nlines, kernel);
+    const int gsize = force_g ? force_g : tuned ? P->tuned_g : g_group_size.load(std::memory_order_relaxed);
+    int waves = force_waves ? force_waves : tuned ? P->tuned_waves : g_block_waves.load(std::memory_order_relaxed);
+    // rows in LDS when the row and the kernel's own LDS fit a CU; a block
+    // too big for that falls back to 4 wavefronts before rows go to HBM
+    auto fits = [&](int wv) {
+        return deposit_lds_bytes(nbins) + deposit_static_lds(tau, gsize, effective_waves(true, g_stats_on, wv)) <=
+               (size_t)dev.max_lds;
+    };
+    bool lds = fits(waves);
+    if (!lds && fits(4)) {
+        waves = 4;
+        lds = true;
+    }
     const int ngroups = (nlines + kMaxLines - 1) / kMaxLines;
 
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -211,9 +224,6 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         FSAMD_HIP(hipEventRecord(P->ev[0], s));
         g_last_kernel_ms = 0;
     }
-    const bool tuned = P->tuned_g > 0 && P->tuned_key == tune_key(nbins, nlines, kernel);
-    const int gsize = force_g ? force_g : tuned ? P->tuned_g : g_group_size.load(std::memory_order_relaxed);
-    const int waves = force_waves ? force_waves : tuned ? P->tuned_waves : g_block_waves.load(std::memory_order_relaxed);
     const long long chunk_key = plain ? kPlainChunk : (sample_stride > 1 ? -sample_stride : (1ll << 62));
     const bool rec = tau && P->keep_windows;
     if (rec) {
@@ -271,7 +281,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         if (tau && (rc = launch_pack_records(pos, vel, temp, h, A.lines, K, P->npart, P->rec.as<PartRec>(), s)))
             return rc;
         A.queue = P->queue.as<int>() + g;
-        // whole-sightline items: a lane group sets up kPairQueue pairs at a
+        // whole-sightline items: a lane group sets up to kPairQueue pairs at a
         // time; the one-launch-per-sightline mode, whose items are short, one
         A.claim = plain ? 1 : kPairQueue;
         A.stats = g_stats_on ? P->stats.as<long long>() : nullptr;
@@ -531,7 +541,8 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
     static thread_local DeviceInfo dev;
     int rc;
     if (dev.ncu == 0 && (rc = device_info(dev))) return rc;
-    if (deposit_lds_bytes(nbins) + 2048 > (size_t)dev.max_lds)
+    if (deposit_lds_bytes(nbins) + adjoint_static_lds(g_group_size.load(std::memory_order_relaxed)) >
+        (size_t)dev.max_lds)
         return fail(FSAMD_ERR_UNSUPPORTED, "adjoint rows must fit in LDS (nbins too large)");
     std::vector<LineDesc> ld(nlines);
     for (int i = 0; i < nlines; ++i) {

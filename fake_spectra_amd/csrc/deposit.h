@@ -17,6 +17,11 @@ struct WorkItem {
 };
 
 size_t deposit_lds_bytes(int nbins);
+// LDS a deposit kernel declares itself (queues, per-group state, tables):
+// the tau kernel at group width gsize and `waves` wavefronts per block, or
+// the column-density kernel; the adjoint kernel at group width gsize.
+size_t deposit_static_lds(bool tau, int gsize, int waves);
+size_t adjoint_static_lds(int gsize);
 int launch_deposit(bool tau, bool lds, int gsize, int waves, const DepositArgs &A, const WorkItem *items, int item_base,
                    int nblocks, int use_queue, hipStream_t s);
 int deposit_blocks_per_cu(bool tau, bool lds, int gsize, int waves, int nbins);

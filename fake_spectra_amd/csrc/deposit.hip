@@ -535,8 +535,11 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
 // The main kernel sets pairs up kQ at a time, one per lane, into a per-group
 // queue of compact records, so that the FP64 set-up arithmetic runs on kQ
 // lanes at once; activating a queued pair only expands its record into the
-// group's PairTau slot.
-constexpr int kQ = kPairQueue;
+// group's PairTau slot.  kQ is 12 for 8-wavefront blocks (1% faster at c1
+// than 8, two blocks still fit a CU's LDS) and 8 for smaller blocks, where 12
+// would cost a resident block per CU.
+template <int kBlock>
+constexpr int queue_depth() { return kBlock >= 512 ? kPairQueue : 8; }
 
 struct PairLite {
     double vel, inv_b, scale, vhigh, aa, inv_nm1;
@@ -549,10 +552,10 @@ struct PairLite {
 };
 static_assert(sizeof(PairLite) == 88, "PairLite layout");
 
-// Claim the next A.claim (<= kQ) pairs of item w for the group, set up one per lane, and
+// Claim the next A.claim (capped at kQ) pairs of item w for the group, set up one per lane, and
 // queue those walked in FP32; FP64 pairs go straight to the fix-up list.
 // Returns the number queued; `more` says whether the item has pairs left.
-template <int kG, bool kStats>
+template <int kG, int kQ, bool kStats>
 __device__ inline int refill_pairs(const DepositArgs &A, const WorkItem &w, int gitem, const LineDesc &L,
                                    const LosInfo &los, unsigned long long *next, PairLite *q, int lane, bool &more,
                                    LaneStats &st)
@@ -921,6 +924,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                        int use_queue)
 {
     constexpr int kGroups = kBlock / kG;
+    constexpr int kQ = queue_depth<kBlock>();
     extern __shared__ double smem[];
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kGroups];
@@ -978,7 +982,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
             if (dir == 0) {
                 if (qi == qn && more) {
-                    qn = refill_pairs<kG, kStats>(A, w, gitem, L, los, &s_next, q, lane, more, st);
+                    qn = refill_pairs<kG, kQ, kStats>(A, w, gitem, L, los, &s_next, q, lane, more, st);
                     qi = 0;
                 }
                 if (qi < qn) {
@@ -1453,6 +1457,39 @@ int launch_voigt_eval(const double *u, long long n, double a, float *out, hipStr
 // Launchers
 
 size_t deposit_lds_bytes(int nbins) { return sizeof(double) * (size_t)nbins; }
+
+// A kernel's static LDS, or a conservative 64 KiB if the runtime cannot say.
+static size_t static_lds(const void *fn)
+{
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, fn) != hipSuccess) {
+        (void)hipGetLastError();
+        return 65536;
+    }
+    return a.sharedSizeBytes;
+}
+
+template <int kG>
+static size_t tau_static_lds(int waves)
+{
+    switch (waves) {
+    case 2: return static_lds((const void *)tau_kernel<kG, true, false, 128>);
+    case 8: return static_lds((const void *)tau_kernel<kG, true, false, 512>);
+    default: return static_lds((const void *)tau_kernel<kG, true, false, 256>);   // the counting variant's too
+    }
+}
+
+size_t deposit_static_lds(bool tau, int gsize, int waves)
+{
+    if (!tau) return static_lds((const void *)colden_kernel<true>);
+    return gsize == 16 ? tau_static_lds<16>(waves) : gsize == 32 ? tau_static_lds<32>(waves) : tau_static_lds<64>(waves);
+}
+
+size_t adjoint_static_lds(int gsize)
+{
+    return static_lds(gsize == 16 ? (const void *)tau_adjoint_kernel<16>
+                      : gsize == 32 ? (const void *)tau_adjoint_kernel<32> : (const void *)tau_adjoint_kernel<64>);
+}
 
 // Raise a kernel's dynamic-LDS limit once, to the largest size requested so far.
 static int lds_limit(const void *fn, size_t shm, size_t &set_to)

@@ -67,7 +67,7 @@ struct LineDesc {
 };
 
 constexpr int kMaxLines = 8;   // lines per deposit launch
-constexpr int kPairQueue = 8;  // pairs a lane group of the tau kernel sets up at once
+constexpr int kPairQueue = 12; // most pairs a lane group of the tau kernel sets up at once
 
 // One particle's inputs to the optical-depth kernels in one 64-byte record
 // (a single cache line per pair instead of one per input array): position,
@@ -112,7 +112,8 @@ struct DepositArgs {
     const LineDesc *lines;      // device [nlines], lines of this launch
     int nlines;
     int nitems;
-    int claim;                  // pairs a lane group claims at once (tau kernel; 1..kPairQueue)
+    int claim;                  // pairs a lane group claims at once (tau kernel; 1..kPairQueue, capped
+                                // at the kernel's queue depth)
     int *queue;                 // work-queue counter (zeroed before launch)
     int *win;                   // optional [nlines][npairs][2] pixels walked per direction
     long long npairs;

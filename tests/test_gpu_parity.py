@@ -192,6 +192,23 @@ def test_rows_wider_than_lds(oracle):
     assert_colden_close(SP._Particle_Interpolate(*cargs), oracle.interpolate(*cargs), "nbins=40000 colden")
 
 
+@pytest.mark.parametrize("nbins,waves", [(19000, 4), (15000, 8)])
+def test_rows_near_the_lds_limit(oracle, nbins, waves):
+    """Rows that fit LDS only without the kernel's own LDS: 19000 pixels go
+    to HBM at 4 wavefronts, 15000 pixels at 8 wavefronts fall back to 4 and
+    stay in LDS."""
+    from fake_spectra_amd import _native as N
+    w = synth.config(0, nlos=3, n_side=12)
+    w.nbins = nbins
+    N.set_block_waves(waves)
+    try:
+        args = w.interp_args(HI_1215, True)
+        got = SP._Particle_Interpolate(*args)
+    finally:
+        N.set_block_waves(4)
+    assert_tau_close(got, oracle.interpolate(*args), "nbins=%d waves=%d" % (nbins, waves))
+
+
 def test_cold_gas_uses_fp64_profile(oracle):
     """Damping parameters above the FP32 profile's range go through FP64."""
     w = synth.config(0, nlos=6, n_side=10)
