@@ -19,7 +19,7 @@ static thread_local int g_timing_on = 0;   // hipEvents around the launches
 static thread_local double g_last_ms = 0;          // whole call
 static thread_local double g_last_kernel_ms = 0;   // deposit kernel launches only
 static std::atomic<int> g_group_size{16};   // pixels per lane group of the tau kernel (16, 32 or 64)
-static std::atomic<int> g_block_waves{4};   // wavefronts per block (per sightline work item): 2, 4 or 8
+static std::atomic<int> g_block_waves{8};   // wavefronts per block (per sightline work item): 2, 4 or 8
 
 void set_error(const std::string &msg) { g_error = msg; }
 

@@ -168,7 +168,8 @@ int fsamd_set_group_size(int g);
 
 /* Tuning knob with no counterpart in the reference: wavefronts per block of
  * the tau kernel, i.e. per (sightline, line) work item (2, 4 or 8; the
- * counting variant runs with 4).  Process-wide; default 4. */
+ * counting variant runs with 4).  Process-wide; default 8, the fastest
+ * untuned choice at 256^3-512^3 particles. */
 int fsamd_set_block_waves(int w);
 
 /* H(a, u) = Re w(u + ia) at n points u (device pointer, FP64) by the deposit

@@ -131,7 +131,7 @@ def test_group_sizes_match_oracle(oracle, g, waves):
                       [dev(w.dens_for(ln)) for ln in w.lines], w.tautail)
     finally:
         N.set_group_size(16)
-        N.set_block_waves(4)
+        N.set_block_waves(8)
     for ln, o in zip(w.lines, outs):
         assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)),
                          "G=%d waves=%d %s" % (g, waves, ln.name))
@@ -205,7 +205,7 @@ def test_rows_near_the_lds_limit(oracle, nbins, waves):
         args = w.interp_args(HI_1215, True)
         got = SP._Particle_Interpolate(*args)
     finally:
-        N.set_block_waves(4)
+        N.set_block_waves(8)
     assert_tau_close(got, oracle.interpolate(*args), "nbins=%d waves=%d" % (nbins, waves))
 
 
