@@ -535,11 +535,12 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
 // The main kernel sets pairs up kQ at a time, one per lane, into a per-group
 // queue of compact records, so that the FP64 set-up arithmetic runs on kQ
 // lanes at once; activating a queued pair only expands its record into the
-// group's PairTau slot.  kQ is 12 for 8-wavefront blocks (1% faster at c1
-// than 8, two blocks still fit a CU's LDS) and 8 for smaller blocks, where 12
-// would cost a resident block per CU.
-template <int kBlock>
-constexpr int queue_depth() { return kBlock >= 512 ? kPairQueue : 8; }
+// group's PairTau slot.  In 8-wavefront blocks kQ is 12 for 16-lane groups
+// (1% faster at c1 than 8; two blocks still fit a CU's LDS) and 16 for wider
+// groups, which are fewer per block (0.7% faster at c2); smaller blocks keep
+// 8, where more would cost a resident block per CU.
+template <int kG, int kBlock>
+constexpr int queue_depth() { return kBlock >= 512 ? (kG >= 32 ? kPairQueue : 12) : 8; }
 
 struct PairLite {
     double vel, inv_b, scale, vhigh, aa, inv_nm1;
@@ -924,7 +925,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                        int use_queue)
 {
     constexpr int kGroups = kBlock / kG;
-    constexpr int kQ = queue_depth<kBlock>();
+    constexpr int kQ = queue_depth<kG, kBlock>();
     extern __shared__ double smem[];
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kGroups];

@@ -67,7 +67,7 @@ struct LineDesc {
 };
 
 constexpr int kMaxLines = 8;   // lines per deposit launch
-constexpr int kPairQueue = 12; // most pairs a lane group of the tau kernel sets up at once
+constexpr int kPairQueue = 16; // most pairs a lane group of the tau kernel sets up at once
 
 // One particle's inputs to the optical-depth kernels in one 64-byte record
 // (a single cache line per pair instead of one per input array): position,
