@@ -32,6 +32,9 @@
 #include "deposit.h"
 #include "wing_table.h"
 
+#include <mutex>
+#include <unordered_map>
+
 namespace fsamd {
 
 __constant__ __attribute__((aligned(16))) float c_gtab[kGTableSize] = FSAMD_G_COEF_INIT;
@@ -1295,16 +1298,13 @@ int launch_tau_adjoint(int gsize, const DepositArgs &A, const WorkItem *items, i
 {
     const size_t shm = deposit_lds_bytes(A.nbins);
     if (gsize == 16) {
-        FSAMD_HIP(hipFuncSetAttribute((const void *)tau_adjoint_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)shm));
+        if (int rc = raise_lds_limit((const void *)tau_adjoint_kernel<16>, shm)) return rc;
         tau_adjoint_kernel<16><<<nblocks, kThreads, shm, s>>>(A, items, 1);
     } else if (gsize == 32) {
-        FSAMD_HIP(hipFuncSetAttribute((const void *)tau_adjoint_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)shm));
+        if (int rc = raise_lds_limit((const void *)tau_adjoint_kernel<32>, shm)) return rc;
         tau_adjoint_kernel<32><<<nblocks, kThreads, shm, s>>>(A, items, 1);
     } else {
-        FSAMD_HIP(hipFuncSetAttribute((const void *)tau_adjoint_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)shm));
+        if (int rc = raise_lds_limit((const void *)tau_adjoint_kernel<64>, shm)) return rc;
         tau_adjoint_kernel<64><<<nblocks, kThreads, shm, s>>>(A, items, 1);
     }
     FSAMD_HIP(hipGetLastError());
@@ -1492,12 +1492,15 @@ size_t adjoint_static_lds(int gsize)
                       : gsize == 32 ? (const void *)tau_adjoint_kernel<32> : (const void *)tau_adjoint_kernel<64>);
 }
 
-// Raise a kernel's dynamic-LDS limit once, to the largest size requested so far.
-static int lds_limit(const void *fn, size_t shm, size_t &set_to)
+int raise_lds_limit(const void *fn, size_t shm)
 {
-    if (shm > set_to) {
+    static std::mutex m;
+    static std::unordered_map<const void *, size_t> limit;
+    std::lock_guard<std::mutex> g(m);
+    size_t &cur = limit[fn];
+    if (shm > cur) {
         FSAMD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        set_to = shm;
+        cur = shm;
     }
     return FSAMD_OK;
 }
@@ -1506,9 +1509,8 @@ template <int kG, bool kLds, bool kStats, int kBlock>
 static int launch_tau_k(const DepositArgs &A, const WorkItem *items, int item_base, int nblocks, int use_queue,
                         size_t shm, hipStream_t s)
 {
-    static size_t set_to = 0;
     int rc;
-    if (kLds && (rc = lds_limit((const void *)tau_kernel<kG, kLds, kStats, kBlock>, shm, set_to))) return rc;
+    if (kLds && (rc = raise_lds_limit((const void *)tau_kernel<kG, kLds, kStats, kBlock>, shm))) return rc;
     tau_kernel<kG, kLds, kStats, kBlock><<<nblocks, kBlock, shm, s>>>(A, items, item_base, use_queue);
     return FSAMD_OK;
 }
@@ -1556,8 +1558,7 @@ int launch_deposit(bool tau, bool lds, int gsize, int waves, const DepositArgs &
         if (rc) return rc;
     } else {
         if (lds) {
-            FSAMD_HIP(hipFuncSetAttribute((const void *)colden_kernel<true>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            if (int rc2 = raise_lds_limit((const void *)colden_kernel<true>, shm)) return rc2;
             colden_kernel<true><<<nblocks, kThreads, shm, s>>>(A, items, item_base, use_queue);
         } else {
             colden_kernel<false><<<nblocks, kThreads, 0, s>>>(A, items, item_base, use_queue);

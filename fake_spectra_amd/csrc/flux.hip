@@ -234,13 +234,12 @@ int fsamd_flux_power(const double *tau, int nlos, int npix, double scale, double
     const int nk = npix / 2 + 1;
     double *per = nullptr;
     FSAMD_HIP(hipMalloc(&per, sizeof(double) * (size_t)nlos * nk));
-    hipError_t e = hipFuncSetAttribute((const void *)flux_dft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)shm);
-    if (e == hipSuccess) {
-        flux_dft_kernel<<<std::min(nlos, 65536), kFluxThreads, shm, s>>>(tau, nlos, npix, scale, 1.0 / mean_flux,
-                                                                         per);
-        e = hipGetLastError();
+    if (int rc = raise_lds_limit((const void *)flux_dft_kernel, shm)) {
+        (void)hipFree(per);
+        return rc;
     }
+    flux_dft_kernel<<<std::min(nlos, 65536), kFluxThreads, shm, s>>>(tau, nlos, npix, scale, 1.0 / mean_flux, per);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         column_sum_kernel<<<(nk + 255) / 256, 256, 0, s>>>(per, nlos, nk, power);
         e = hipGetLastError();

@@ -127,6 +127,10 @@ struct DepositArgs {
 };
 
 // Host-side launchers (return FSAMD_OK or an error code).
+// Raise kernel fn's dynamic-LDS limit to shm bytes if it is lower.  The limit
+// only grows, under a lock, so a launch made after its own call here stays
+// valid whatever sizes other host threads ask for meanwhile.
+int raise_lds_limit(const void *fn, size_t shm);
 int launch_fill_i32(int *p, int value, long long n, hipStream_t s);
 int launch_fill_f64(double *p, double value, long long n, hipStream_t s);
 int launch_pack_records(const float *pos, const float *vel, const float *temp, const float *h,

@@ -209,6 +209,27 @@ def test_rows_near_the_lds_limit(oracle, nbins, waves):
     assert_tau_close(got, oracle.interpolate(*args), "nbins=%d waves=%d" % (nbins, waves))
 
 
+def test_concurrent_calls_match_sequential():
+    """Drop-in calls from several host threads at once, each with its own
+    pixel count (and so its own LDS size), give the results of the same calls
+    made one after another."""
+    from concurrent.futures import ThreadPoolExecutor
+    w = synth.config(0, nlos=8, n_side=16)
+    sizes = (512, 1024, 2048, 4096, 3000, 700)
+
+    def run(nbins):
+        args = list(w.interp_args(HI_1215, True))
+        args[1] = nbins
+        return SP._Particle_Interpolate(*args)
+
+    want = [run(n) for n in sizes]
+    for _ in range(3):
+        with ThreadPoolExecutor(max_workers=len(sizes)) as ex:
+            got = list(ex.map(run, sizes))
+        for n, a, b in zip(sizes, got, want):
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=0, err_msg="nbins=%d" % n)
+
+
 def test_cold_gas_uses_fp64_profile(oracle):
     """Damping parameters above the FP32 profile's range go through FP64."""
     w = synth.config(0, nlos=6, n_side=10)
