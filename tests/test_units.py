@@ -23,10 +23,21 @@ def test_velfac_is_the_hubble_flow_across_a_comoving_kpc():
     assert abs(synth.Cosmology().velfac() - 0.111524) < 1e-6
 
 
+def _rho_crit(hubble):
+    """3 H0^2 / (8 pi G) in g/cm^3."""
+    h0 = 100 * hubble * 1e5 / MPC_CM                              # 1/s
+    return 3 * h0 ** 2 / (8 * math.pi * G_CGS)
+
+
+def test_critical_density_known_answers():
+    """The reference's own known answers (tests/test_spectra.py:93-97)."""
+    assert abs(_rho_crit(0.7) / 9.204285430050004e-30 - 1) < 1e-6
+    assert abs(_rho_crit(1.0) / 1.8784255979693885e-29 - 1) < 1e-6
+
+
 def test_gas_density_is_omega_b_times_the_critical_density():
     c = synth.Cosmology()
-    h0 = 100 * c.hubble * 1e5 / MPC_CM                            # 1/s
-    rho_crit = 3 * h0 ** 2 / (8 * math.pi * G_CGS)                # g/cm^3 today
+    rho_crit = _rho_crit(c.hubble)                                # g/cm^3 today
     n_mean = c.omega_b * rho_crit * (1 + c.redshift) ** 3 / PROTONMASS
     w = synth.config(0, scale=1 / 64)
     # the field's own mean: den = rho * density_conv with rho = Omega_b rho_crit (1 + delta)
