@@ -16,8 +16,9 @@
 //            follows the asymptotic series w ~ (i/sqrt(pi)) sum_n (2n-1)!!/2^n
 //            z^-(2n+1); Z = x^2 Q(x), Q = sum C(2n+3,3) (2n-1)!!/2^n x^n (3 terms),
 //            the a^3 term of the same series, free of the cancellation in D'''.
-//  Both forms are evaluated by every lane and selected, so the wavefront
-//  never diverges on |u|.
+//  Each lane picks its table row (a core piece or the wing row) and its a^3
+//  form by selects, and every lane runs the same instructions, so the
+//  wavefront never diverges on |u|.
 //
 // u arrives in FP64, and u^2 and the exponent of exp(a^2 - u^2) stay in FP64
 // because exp amplifies the relative error of u by 2u^2; only 2^f on
