@@ -4,11 +4,14 @@
 A step is one pass of the hot path over the workload: build the
 sightline<->particle neighbour lists (IndexTable) from device-resident
 particles, then deposit the optical depth of every line of every sightline
-in one batched launch.  The default workload is 256^3 particles in a
-60 Mpc/h box, 5000 sightlines of 2048 pixels, H I 1215 and H I 1025
-(synthetic gas field, seeded).  Prints one JSON line.
+in one batched launch.  The default workload is 256^3 clustered particles
+in a 60 Mpc/h box (70% uniform, 30% in 2000 Gaussian blobs), 3 x 224^2
+gridded sightlines along the three axes, 1024 pixels each, H I 1215, H I 1025
+and C IV 1548 (synthetic gas field, seeded).  Prints one JSON line, which
+also carries the parity of the timed rows against the reference's rows of
+the CPU baseline's sightlines.
 
-    python bench.py [--steps K] [--warmup W] [--config 1|2|...] [--no-cpu] [--no-plain]
+    python bench.py [--steps K] [--warmup W] [--config 0..4] [--no-cpu] [--no-plain]
 
 Defaults: 3 untimed warm-up steps, 10 timed steps (value from their total
 time; the median step is reported beside it).
@@ -40,11 +43,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2,
+                    help="workload (2: 256^3 clustered, 3 x 224^2 sightlines x 1024 pixels, 3 lines)")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget of the _near_lines host sample and of the whole-job table sample")
+    ap.add_argument("--cpu-sightlines", type=int, default=1000,
+                    help="sightlines of the CPU baseline's deposit sample (and of the parity check)")
+    ap.add_argument("--graph-replays", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-near-lines", action="store_true")
     ap.add_argument("--no-drop-in", action="store_true")
@@ -70,40 +78,87 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "copies": 3}))
 
 
-def cpu_baseline(w, budget_s):
-    """The reference C++ (oracle/_ref, OpenMP, all host threads) on every k-th
-    sightline of the same workload with the full particle set, one
-    _Particle_Interpolate call per line as the reference makes them
-    (spectra.py:673), the sample grown until about budget_s of CPU time.
-    Falls back to the oracle's C restatement where the reference library is
-    not built."""
+def cpu_sample(w, nsel):
+    """The sightlines the CPU baseline runs: every k-th, at least nsel of them."""
+    import numpy as np
+    k = max(1, w.nlos // max(1, nsel))
+    return np.arange(0, w.nlos, k)[:max(nsel, 1)], k
+
+
+def cpu_baseline(w, sel, k, table_budget_s):
+    """The reference C++ (oracle/_ref, OpenMP, all host threads; the oracle's
+    C restatement where the reference library is not built) on the full
+    particle set, one _Particle_Interpolate call per line as the reference
+    makes them (spectra.py:673), scaled to the whole job.
+
+    A reference call is IndexTable + get_near_particles over all particles
+    and the call's sightlines (index_table.cpp:130-150; its pair inserts are
+    serialised under omp critical), then the per-sightline deposit
+    (part_int.cpp:20-51).  Both are estimated separately:
+      * the deposit from the sampled sightlines: each sampled call's time
+        minus the time of its table alone (measured on the same sightlines);
+      * the table of a whole-job call (all sightlines, so all M pairs) from
+        every kp-th particle with all sightlines, times kp, kp grown until
+        that sample takes about table_budget_s.
+    Job time = per line: table(all) + deposit(sample) x L / L_sample.
+    Returns the baseline and the sampled rows (for the parity check)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     kind = "reference" if O.ref_available() else "port"
     lib = O.load_ref() if kind == "reference" else O.load_oracle()
     cores = lib.max_threads()
-    nsel = 4
-    t = 0.0
+    rows = []
+    t_calls = []
+    for line in w.lines:
+        args = list(w.interp_args(line, True))
+        args[16], args[17] = w.axis[sel], w.cofm[sel]
+        t0 = time.perf_counter()
+        rows.append(lib.interpolate(*args))
+        t_calls.append(time.perf_counter() - t0)
+        progress("cpu baseline: %s on %d sightlines, %.1f s" % (line.name, len(sel), t_calls[-1]))
+    t_tab_s, m_s = lib.table_seconds(w.box, w.pos, w.hh, w.axis[sel], w.cofm[sel])
+    kp = 1024
     while True:
-        k = max(1, w.nlos // nsel)
-        sel = np.arange(0, w.nlos, k)[:nsel]
-        t = 0.0
-        for line in w.lines:
-            args = list(w.interp_args(line, True))
-            args[16], args[17] = w.axis[sel], w.cofm[sel]
-            t0 = time.perf_counter()
-            lib.interpolate(*args)
-            t += time.perf_counter() - t0
-        if t >= budget_s / 3 or nsel >= w.nlos:
+        kp = max(1, kp)
+        t_tab_p, m_p = lib.table_seconds(w.box, w.pos[::kp], w.hh[::kp], w.axis, w.cofm)
+        if t_tab_p >= table_budget_s / 4 or kp == 1:
             break
-        nsel = min(w.nlos, max(nsel * 2, int(nsel * budget_s / 3 / max(t, 1e-3))))
-    value = len(w.lines) * len(sel) * w.nbins / t
-    return {"value": value, "unit": "sightline-pixels/s", "cores": cores, "kind": kind,
-            "sample": "%s, lines %s (one call each), %d of %d sightlines (every %d-th), all %d particles, "
-                      "index table + deposit, %.1f s" % (
-                          "reference C++ -O2 -ffast-math OpenMP" if kind == "reference" else "C restatement",
-                          "+".join(ln.name for ln in w.lines), len(sel), w.nlos, k, w.npart, t)}
+        kp = max(1, int(kp * t_tab_p / table_budget_s) if t_tab_p > 0.05 else kp // 8)
+    t_tab_full = t_tab_p * kp
+    dep = [max(tc - t_tab_s, 0.0) for tc in t_calls]
+    job_s = sum(t_tab_full + d * w.nlos / len(sel) for d in dep)
+    units = len(w.lines) * w.nlos * w.nbins
+    value = units / job_s
+    sample = ("%s, lines %s (one call each, all %d particles): deposit from %d of %d sightlines (every %d-th), "
+              "%.1f s of calls less %.2f s of table per call; whole-job table (all sightlines, %d pairs) from "
+              "every %d-th particle (%d pairs, %.2f s) x %d = %.1f s per call; job estimate %.0f s" % (
+                  "reference C++ -O2 -ffast-math OpenMP" if kind == "reference" else "C restatement",
+                  "+".join(ln.name for ln in w.lines), w.npart, len(sel), w.nlos, k, sum(t_calls), t_tab_s,
+                  m_p * kp, kp, m_p, t_tab_p, kp, t_tab_full, job_s))
+    return {"value": value, "unit": "sightline-pixels/s", "cores": cores, "kind": kind, "sample": sample,
+            "sightlines_sampled": int(len(sel)), "job_seconds_estimate": job_s,
+            "table_seconds_per_call": t_tab_full,
+            "deposit_seconds_per_call": [d * w.nlos / len(sel) for d in dep]}, rows
+
+
+def parity_rows(w, gpu_rows, ref_rows):
+    """Timed GPU rows against the CPU baseline's rows for the same sightlines,
+    at the stated tolerances (relative tau where tau > 1e-3, flux absolute)."""
+    import numpy as np
+    tau_rel = 0.0
+    flux = 0.0
+    zero_ok = True
+    for g, r in zip(gpu_rows, ref_rows):
+        m = r > 1e-3
+        if m.any():
+            tau_rel = max(tau_rel, float(np.max(np.abs(g[m] - r[m]) / r[m])))
+        flux = max(flux, float(np.max(np.abs(np.exp(-g) - np.exp(-r)))))
+        zero_ok = zero_ok and bool(np.all(g[r == 0] == 0))
+    return {"sightlines": int(gpu_rows[0].shape[0]), "lines": [ln.name for ln in w.lines],
+            "tau_rel_max": tau_rel, "flux_abs_max": flux, "zero_pixels_stay_zero": zero_ok,
+            "tolerance": {"tau_rel": 1e-5, "tau_floor": 1e-3, "flux_abs": 1e-6},
+            "pass": tau_rel <= 1e-5 and flux <= 1e-6 and zero_ok}
 
 
 def near_lines_leg(w, pos, hh, ax, cofm, with_cpu, budget_s):
@@ -159,16 +214,12 @@ def drop_in_leg(w):
     """The numpy-in, numpy-out _Particle_Interpolate call (the reference's
     boundary, py_module.cpp:103-233), one per line as spectra.py:673 makes
     them: host-to-device copies of the particle arrays, binning, deposit and
-    the copy of the result back to the host, all timed (median of 3)."""
+    the copy of the result back to the host, all timed (one pass)."""
     from fake_spectra_amd import spectra_priv as SP
-    SP._Particle_Interpolate(*w.interp_args(w.lines[0], True))
-    ts = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        for ln in w.lines:
-            SP._Particle_Interpolate(*w.interp_args(ln, True))
-        ts.append(time.perf_counter() - t0)
-    t = sorted(ts)[1]
+    t0 = time.perf_counter()
+    for ln in w.lines:
+        SP._Particle_Interpolate(*w.interp_args(ln, True))
+    t = time.perf_counter() - t0
     return {"value": len(w.lines) * w.nlos * w.nbins / t, "unit": "sightline-pixels/s", "ms_per_step": 1e3 * t,
             "note": "host numpy in and out (PCIe included), one call per line"}
 
@@ -298,6 +349,7 @@ def main():
     dens = [dev(w.dens_for(ln)) for ln in w.lines]
     outs = [torch.empty((w.nlos, w.nbins), dtype=torch.float64, device="cuda") for _ in w.lines]
     K = len(w.lines)
+    sel, sel_k = cpu_sample(w, args.cpu_sightlines)
     torch.cuda.synchronize()
 
     # pick the lane-group width on this workload (kept by the plan for later calls)
@@ -340,6 +392,9 @@ def main():
     dep_ms = float(np.mean(state["ms"]))              # deposit call: fills + tau kernel + FP64 fix-up
     ker_ms = float(np.mean(state["kernel_ms"]))       # the tau kernel launches alone
     progress("timed steps: %.1f ms per step" % (1e3 * elapsed / args.steps))
+    # the timed output rows of the sightlines the CPU baseline runs (parity check)
+    sel_t = torch.as_tensor(sel, device="cuda")
+    gpu_rows = [o[sel_t].cpu().numpy() for o in outs]
     SP.set_stats(True)
     step()
     SP.set_stats(False)
@@ -406,11 +461,12 @@ def main():
     graph.replay()
     torch.cuda.synchronize()
     ev0.record()
-    for _ in range(args.steps):
+    for _ in range(args.graph_replays):
         graph.replay()
     ev1.record()
     torch.cuda.synchronize()
-    out["graph"] = {"deposit_ms": ev0.elapsed_time(ev1) / args.steps, "overflow": graph.overflowed(),
+    out["graph"] = {"deposit_ms": ev0.elapsed_time(ev1) / max(1, args.graph_replays),
+                    "replays": args.graph_replays, "overflow": graph.overflowed(),
                     "note": "batched deposit + fix-up pass replayed from one captured hipGraph (binning excluded)"}
     del graph
     gplan.close()
@@ -418,6 +474,11 @@ def main():
     out["autotune"] = {"group_width": best_g, "waves_per_sightline": best_w,
                        "sample_ms": {"%dx%d" % k: v for k, v in tune_ms.items()}}
     if not args.no_plain:
+        # the plain form at the library's default lane-group width and block
+        # size (16 lanes, 8 wavefronts), whatever the autotuner picked for the
+        # batched launch, so that its time does not move with the tuning
+        N.set_group_size(16)
+        N.set_block_waves(8)
         plan = SP.Plan(w.box, pos, hh, ax, cofm)
         # one untimed call builds the per-sightline work items
         SP.tau(plan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail, out=outs, mode=N.MODE_PLAIN)
@@ -428,9 +489,12 @@ def main():
         tplain = time.perf_counter() - t0
         tbatch = dep_ms * 1e-3
         out["plain"] = {"value": units / tplain, "deposit_s": tplain,
-                        "batched_over_plain": tplain / tbatch,
-                        "note": "deposit only, same binning; one launch per sightline and line"}
+                        "batched_over_plain": tplain / tbatch, "group_width": 16, "waves_per_sightline": 8,
+                        "note": "deposit only, same binning; one launch per sightline and line, fixed "
+                                "16-lane groups and 8-wavefront blocks"}
         plan.close()
+        N.set_group_size(best_g)
+        N.set_block_waves(best_w)
         progress("plain: %.2f s" % tplain)
     if not args.no_drop_in:
         out["drop_in"] = drop_in_leg(w)
@@ -446,10 +510,13 @@ def main():
         out["adjoint"] = adjoint_leg(max(2, args.steps), args.adjoint_config, w if same else None)
         progress("adjoint leg done")
     if not args.no_cpu:
-        cb = cpu_baseline(w, args.cpu_seconds)
+        cb, ref_rows = cpu_baseline(w, sel, sel_k, args.cpu_seconds)
         cb["host"] = host_cpu()
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
+        out["parity"] = parity_rows(w, gpu_rows, ref_rows)
+        out["parity"]["against"] = "cpu_baseline rows (%s) of the same sightlines" % cb["kind"]
+        progress("cpu baseline and parity done: %s" % json.dumps(out["parity"]))
     print(json.dumps(out))
 
 

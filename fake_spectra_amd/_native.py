@@ -27,7 +27,7 @@ EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan
            "fsamd_voigt_peak", "fsamd_set_group_size", "fsamd_plan_keep_windows", "fsamd_tau_adjoint",
            "fsamd_tau_autotune", "fsamd_plan_overflow", "fsamd_mean_flux_scale", "fsamd_res_corr",
            "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms", "fsamd_set_block_waves",
-           "fsamd_voigt_eval")
+           "fsamd_voigt_eval", "fsamd_plan_window_generation", "fsamd_set_fp64")
 
 
 class FsamdLine(C.Structure):
@@ -86,6 +86,10 @@ def _declare(L):
     L.fsamd_set_block_waves.restype = i
     L.fsamd_plan_keep_windows.argtypes = [vp, i]
     L.fsamd_plan_keep_windows.restype = i
+    L.fsamd_plan_window_generation.argtypes = [vp, C.POINTER(ll)]
+    L.fsamd_plan_window_generation.restype = i
+    L.fsamd_set_fp64.argtypes = [i]
+    L.fsamd_set_fp64.restype = i
     L.fsamd_tau_adjoint.argtypes = [vp, i, i, d, vp, vp, vp, vp, C.POINTER(FsamdLineGrad), i, vp, vp, vp]
     L.fsamd_tau_adjoint.restype = i
     L.fsamd_tau_autotune.argtypes = [vp, i, i, d, d, vp, vp, vp, vp, C.POINTER(FsamdLine), i, vp, C.POINTER(i),

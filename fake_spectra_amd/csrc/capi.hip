@@ -20,6 +20,7 @@ static thread_local double g_last_ms = 0;          // whole call
 static thread_local double g_last_kernel_ms = 0;   // deposit kernel launches only
 static std::atomic<int> g_group_size{16};   // pixels per lane group of the tau kernel (16, 32 or 64)
 static std::atomic<int> g_block_waves{8};   // wavefronts per block (per sightline work item): 2, 4 or 8
+static thread_local int g_fp64 = 0;         // every pair on the FP64 path (fsamd_set_fp64)
 
 void set_error(const std::string &msg) { g_error = msg; }
 
@@ -237,14 +238,16 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         P->win_lines = nlines;
         P->win_nbins = nbins;
         P->win_kernel = kernel;
+        ++P->win_gen;
     }
     if (tau) {
         // room for deferred FP64 walks: a few percent of pair-directions
-        const long long want = std::max<long long>(65536, 2 * P->npairs * std::min(nlines, kMaxLines) / 32);
+        const long long want = g_fp64 ? P->npairs * std::min(nlines, kMaxLines) + 65536
+                                      : std::max<long long>(65536, 2 * P->npairs * std::min(nlines, kMaxLines) / 32);
         if ((P->fix_cap < want || !P->fix_count.ptr) && capturing)
             return fail(FSAMD_ERR_ARG, "run one uncaptured fsamd_tau call before capturing");
         if (P->fix_cap < want && (rc = grow_fixups(P, want, s))) return rc;
-        if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(int) * 2))) return rc;
+        if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(unsigned long long) * 2))) return rc;
     }
     if (tau) {
         const size_t rbytes = sizeof(PartRec) * (size_t)std::max<long long>(P->npart, 1);
@@ -267,6 +270,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         A.vbox = P->box * velfac;     // absorption.cpp:157
         A.box = P->box;
         A.tautail = tautail;
+        A.force_fp64 = g_fp64;
         A.pos = pos;
         A.vel = vel;
         A.temp = temp;
@@ -295,14 +299,14 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         for (int attempt = 0;; ++attempt) {
             A.fix = P->fix.as<Fixup>();
             A.fix_cap = P->fix_cap;
-            A.fix_count = P->fix_count.as<int>();
+            A.fix_count = P->fix_count.as<unsigned long long>();
             if (set->split || !lds) {
                 for (int k = 0; k < K; ++k)
                     if ((rc = launch_fill_f64(ld[k0 + k].out, 0.0, (long long)P->nlos * nbins, s))) return rc;
             }
             if (items.empty()) break;
             // captured graphs keep the overflow flag sticky across replays
-            if (tau && (rc = launch_fill_i32(A.fix_count, 0, capturing ? 1 : 2, s))) return rc;
+            if (tau && (rc = launch_fill_i32((int *)A.fix_count, 0, capturing ? 2 : 4, s))) return rc;
             if (g_timing_on) FSAMD_HIP(hipEventRecord(P->ev[2], s));
             if (!plain) {
                 A.nitems = (int)items.size();
@@ -334,13 +338,16 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
             if (!tau) break;
             if ((rc = launch_tau_fixup(A, set->buf.as<WorkItem>(), dev.ncu * 2, s))) return rc;
             if (capturing) break;   // overflow is checked after replay (fsamd_plan_overflow)
-            int fc[2] = {0, 0};
-            FSAMD_HIP(hipMemcpyAsync(fc, A.fix_count, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+            unsigned long long fc[2] = {0, 0};
+            FSAMD_HIP(hipMemcpyAsync(fc, A.fix_count, sizeof(fc), hipMemcpyDeviceToHost, s));
             FSAMD_HIP(hipStreamSynchronize(s));
             if (!fc[1]) break;
             // the list overflowed: grow it and run this line group again
             if (attempt > 2) return fail(FSAMD_ERR_HIP, "fix-up list overflow persists");
-            if ((rc = grow_fixups(P, 2ll * fc[0] + 65536, s))) return rc;
+            if (fc[0] > (unsigned long long)(INT32_MAX / 2))
+                return fail(FSAMD_ERR_UNSUPPORTED, "more than 2^30 walks of one line group need the FP64 path; "
+                                                   "split the sightlines over several plans");
+            if ((rc = grow_fixups(P, 2ll * (long long)fc[0] + 65536, s))) return rc;
             if (g_stats_on && (rc = launch_fill_f64((double *)P->stats.ptr, 0.0, kNStats, s))) return rc;
             if (g_timing_on) {
                 float kms = 0;
@@ -460,10 +467,10 @@ int fsamd_plan_overflow(fsamd_plan *plan, int *flag, void *stream)
     if (!plan || !flag) return fail(FSAMD_ERR_ARG, "NULL argument");
     *flag = 0;
     if (!plan->fix_count.ptr) return FSAMD_OK;
-    int fc[2] = {0, 0};
-    FSAMD_HIP(hipMemcpyAsync(fc, plan->fix_count.ptr, sizeof(int) * 2, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    unsigned long long fc[2] = {0, 0};
+    FSAMD_HIP(hipMemcpyAsync(fc, plan->fix_count.ptr, sizeof(fc), hipMemcpyDeviceToHost, (hipStream_t)stream));
     FSAMD_HIP(hipStreamSynchronize((hipStream_t)stream));
-    *flag = fc[1];
+    *flag = fc[1] != 0;
     return FSAMD_OK;
 }
 
@@ -522,9 +529,21 @@ int fsamd_plan_keep_windows(fsamd_plan *plan, int enable)
     if (!plan) return fail(FSAMD_ERR_ARG, "plan is NULL");
     plan->keep_windows = enable != 0;
     if (!enable) {
-        plan->win.release();
+        // a graph captured while recording was on still writes this buffer
+        if (plan->captured && plan->win.ptr)
+            plan->retired.push_back(plan->win.disown());
+        else
+            plan->win.release();
         plan->win_lines = 0;
+        ++plan->win_gen;
     }
+    return FSAMD_OK;
+}
+
+int fsamd_plan_window_generation(const fsamd_plan *plan, long long *gen)
+{
+    if (!plan || !gen) return fail(FSAMD_ERR_ARG, "NULL argument");
+    *gen = plan->win_gen;
     return FSAMD_OK;
 }
 
@@ -577,6 +596,7 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         A.inv_velfac = 1.0 / velfac;
         A.vbox = P->box * velfac;
         A.box = P->box;
+        A.force_fp64 = g_fp64;
         A.pos = pos;
         A.vel = vel;
         A.temp = temp;
@@ -622,6 +642,12 @@ int fsamd_last_stats(long long *out, int n)
 
 double fsamd_last_ms(void) { return g_last_ms; }
 double fsamd_last_kernel_ms(void) { return g_last_kernel_ms; }
+
+int fsamd_set_fp64(int on)
+{
+    g_fp64 = on != 0;
+    return FSAMD_OK;
+}
 
 int fsamd_set_group_size(int g)
 {

@@ -389,13 +389,15 @@ __device__ inline StepShape step_shape(int np, int kg)
     return sh;
 }
 
+// Append a walk to the fix-up list.  The counter is 64-bit, so it cannot wrap
+// within a launch; entries past the capacity only raise the overflow flag.
 __device__ inline void push_fixup(const DepositArgs &A, long long k, int item, int z0, int count, int dir)
 {
-    const int slot = atomicAdd(&A.fix_count[0], 1);
-    if (slot < A.fix_cap)
+    const unsigned long long slot = atomicAdd(&A.fix_count[0], 1ull);
+    if (slot < (unsigned long long)A.fix_cap)
         A.fix[slot] = Fixup{k, item, z0, count, dir};
     else
-        atomicOr(&A.fix_count[1], 1);
+        atomicOr(&A.fix_count[1], 1ull);
 }
 
 // Lanes of a wave are split into groups of kG lanes; each group walks its own
@@ -463,7 +465,7 @@ __device__ inline bool pair_consts(const DepositArgs &A, int line, const LineDes
     // FP64 path, which follows the reference pixel by pixel.
     const bool grid_edge = fabs(bintov - b / 2.) <= 1e-12 * bintov || (np > 1 && fabs(xs - rint(xs)) <= 1e-12 * xs);
     // pixels wider than max_sums sub-samples do not fit one walk step
-    C.fp64 = !(aa <= (double)kA32Max) || grid_edge || np - 1 > max_sums - 1;
+    C.fp64 = A.force_fp64 || !(aa <= (double)kA32Max) || grid_edge || np - 1 > max_sums - 1;
     const double dmax = (vhigh - dvk) * inv_b + (np > 1 ? 0.5 * bintov * inv_b : 0.0);
     C.ufar = fmax(7.0 + dmax, 3.0 * dmax);
     C.vel = vel;
@@ -1078,7 +1080,7 @@ __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, cons
     const int lane = threadIdx.x & 63, gid = threadIdx.x / kFixG, glane = lane % kFixG;
     PairTau &P = pslot[gid];
     LaneStats st;
-    const int n = min(A.fix_count[0], A.fix_cap);
+    const int n = (int)min(A.fix_count[0], (unsigned long long)A.fix_cap);
     const int half = A.nbins / 2;
     for (int e = blockIdx.x * kGroups + gid; e < n; e += gridDim.x * kGroups) {
         const Fixup f = A.fix[e];
@@ -1140,6 +1142,54 @@ __device__ inline void adj_inner32(const PairTau &P, double vouter, const float 
         SU = __builtin_fmaf(w, g.Hu, SU);
         SF = __builtin_fmaf(w, g.F, SF);
     }
+}
+
+// The same three sums with the FP64 Faddeeva function and FP64 kernel weights,
+// for the pairs the forward evaluates in FP64 (damping parameter above
+// kA32Max, or a sub-sample count on a switch point).  With L = Im w:
+//   dH/du = -2(uH - aL),  u dH/du + a dH/da = -2(u^2 - a^2)H + 4auL - 2a/sqrt(pi).
+__device__ __noinline__ void adj_inner64(double aa, double b, double dvk, double vhigh, double vdr2, double vsmooth,
+                                         double vouter, int kernel, double &S, double &SU, double &SF)
+{
+    for (int i = 1; i < kNGrid; ++i) {
+        const double vv = i * dvk - vhigh;
+        const double W = kernel_weight(kernel, sqrt(vdr2 + vv * vv) / vsmooth);
+        const double u = (vv - vouter) / b;
+        double H, L;
+        w64(u, aa, H, L);
+        S += W * H;
+        SU += W * (-2.0 * (u * H - aa * L));
+        SF += W * (-2.0 * (u * u - aa * aa) * H + 4.0 * aa * u * L - aa * kTwoOverSqrtPi);
+    }
+}
+
+// One pixel of an FP64 pair: the sums averaged over the pixel's own
+// sub-sample grid, as tau_kern_outer builds it (singleabs.h:104-126) and as
+// the forward's FP64 walk evaluates it.
+__device__ inline void adj_pixel64(const PairTau &P, double vlow, double bintov, int kernel, double &S, double &SU,
+                                   double &SF)
+{
+    const double vhighp = vlow + bintov;
+    const double width = vhighp - vlow;
+    S = SU = SF = 0.0;
+    if (width < P.b / 2.) {
+        adj_inner64(P.aa, P.b, P.dvk, P.vhigh, P.vdr2, P.vsmooth, (vhighp + vlow) / 2., kernel, S, SU, SF);
+        return;
+    }
+    const int np = 2 * ceil(width / (P.b / 2.) / 2) + 1.;
+    const double dv = width / (np - 1);
+    double s0 = 0, u0 = 0, f0 = 0;
+    adj_inner64(P.aa, P.b, P.dvk, P.vhigh, P.vdr2, P.vsmooth, vlow, kernel, s0, u0, f0);
+    adj_inner64(P.aa, P.b, P.dvk, P.vhigh, P.vdr2, P.vsmooth, vhighp, kernel, s0, u0, f0);
+    S = 0.5 * s0;
+    SU = 0.5 * u0;
+    SF = 0.5 * f0;
+    for (int i = 1; i < np - 1; ++i)
+        adj_inner64(P.aa, P.b, P.dvk, P.vhigh, P.vdr2, P.vsmooth, i * dv + vlow, kernel, S, SU, SF);
+    const double inv = 1.0 / (np - 1);
+    S *= inv;
+    SU *= inv;
+    SF *= inv;
 }
 
 // Far-wing pixel (|U| >= ufar) of the adjoint: S, SU and SF by the moment
@@ -1219,6 +1269,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             const int n = lf + lb;
             const int zlo = P.zmax - lb;   // the window: pixels zlo .. zlo + n - 1
             double aS = 0, aU = 0, aF = 0;
+            if (P.fp64) {
+                // the forward walked this pair with the FP64 profile, pixel by pixel
+                for (int j = glane; j < n; j += kG) {
+                    const int z = zlo + j;
+                    double S, SU, SF;
+                    adj_pixel64(P, z * bintov - P.vel, bintov, A.kernel, S, SU, SF);   // absorption.cpp:252
+                    const double g = grow[wrap_pixel(z, A.nbins)];
+                    aS += g * S;
+                    aU += g * SU;
+                    aF += g * SF;
+                }
+                aS = group_sum<kG>(aS);
+                aU = group_sum<kG>(aU);
+                aF = group_sum<kG>(aF);
+                if (glane == 0 && n > 0) {
+                    const int p = P.part;
+                    unsafeAtomicAdd(&L.grad_dens[p], P.scale_nd * aS);
+                    unsafeAtomicAdd(&A.grad_vel[3 * (size_t)p + los.ax - 1], P.scale * P.inv_b * aU);
+                    unsafeAtomicAdd(&A.grad_temp[p], -P.scale / (2.0 * P.temp) * (aS + aF));
+                }
+                continue;
+            }
             // far-wing pixels (|U| >= ufar) sit at the two ends of the window
             int nfl = 0, nfr = 0;
             for (int c = 0; c < n; c += kG) {

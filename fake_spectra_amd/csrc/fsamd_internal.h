@@ -121,8 +121,9 @@ struct DepositArgs {
     double *grad_temp;          // [N] +=
     double *grad_vel;           // [N,3] +=
     Fixup *fix;                 // fix-up list
-    int *fix_count;             // [0] entries appended, [1] overflow flag
+    unsigned long long *fix_count;   // [0] entries appended (64-bit: cannot wrap), [1] overflow flag
     int fix_cap;
+    int force_fp64;             // every pair on the FP64 path (fsamd_set_fp64)
     long long *stats;           // optional counters, kNStats entries (see fsamd_last_stats)
 };
 

@@ -83,13 +83,14 @@ struct fsamd_plan {
     fsamd::DevBuf stats;        // kNStats counters
     fsamd::DevBuf rec;          // [npart] packed particle records of the current launch
     fsamd::DevBuf fix;          // fix-up list (FP64 walks)
-    fsamd::DevBuf fix_count;    // [count, overflow] per line group
+    fsamd::DevBuf fix_count;    // [count, overflow] (uint64 each)
     int fix_cap = 0;
     // pixels walked per (line, pair, direction) by the last fsamd_tau call,
     // kept for the adjoint when keep_windows is set
     bool keep_windows = false;
     fsamd::DevBuf win;
     int win_lines = 0, win_nbins = 0, win_kernel = -1;
+    long long win_gen = 0;      // bumped whenever the recorded windows change
     // lane-group width chosen by fsamd_tau_autotune for (nbins, nlines, kernel)
     int tuned_g = 0;
     int tuned_waves = 4;

@@ -116,10 +116,36 @@ class Plan:
         p = torch.as_tensor(_DeviceArray(parts.value, (m,), "<i4"), device=self.pos.device).clone()
         return o, p
 
+    def offsets(self) -> torch.Tensor:
+        """Row offsets int64 [nlos+1] of the neighbour lists (a device copy)."""
+        off = C.c_void_p()
+        N.check(N.lib().fsamd_plan_pairs(self.handle, C.byref(off), None))
+        torch.cuda.current_stream().synchronize()
+        return torch.as_tensor(_DeviceArray(off.value, (self.nlos + 1,), "<i8"), device=self.pos.device).clone()
+
+    def rows(self, sightlines) -> list:
+        """Particle ids (numpy int32) of the given sightlines' neighbour lists,
+        copying only those rows to the host."""
+        off, parts = C.c_void_p(), C.c_void_p()
+        N.check(N.lib().fsamd_plan_pairs(self.handle, C.byref(off), C.byref(parts)))
+        torch.cuda.current_stream().synchronize()
+        o = self.offsets().cpu().numpy()
+        m = self.npairs
+        if m == 0:
+            return [np.zeros(0, dtype=np.int32) for _ in sightlines]
+        p = torch.as_tensor(_DeviceArray(parts.value, (m,), "<i4"), device=self.pos.device)
+        return [p[int(o[s]):int(o[s + 1])].cpu().numpy() for s in sightlines]
+
     def keep_windows(self, enable: bool = True):
         """Record the deposited window of every (line, pair, direction) in the
         next tau() call, as the adjoint needs."""
         N.check(N.lib().fsamd_plan_keep_windows(self.handle, 1 if enable else 0))
+
+    def window_generation(self) -> int:
+        """Counter bumped whenever the recorded windows change."""
+        g = C.c_longlong()
+        N.check(N.lib().fsamd_plan_window_generation(self.handle, C.byref(g)))
+        return int(g.value)
 
     def near_lines(self) -> torch.Tensor:
         """Sorted ids of particles near any sightline (_near_lines)."""
@@ -281,6 +307,12 @@ def voigt_profile(u, a: float) -> torch.Tensor:
     out = torch.empty(uu.numel(), dtype=torch.float32, device=dev)
     N.check(N.lib().fsamd_voigt_eval(_ptr(uu), uu.numel(), float(a), _ptr(out), _stream()))
     return out
+
+
+def set_fp64(on: bool):
+    """Evaluate every pair of later tau / tau_adjoint calls on this thread with
+    the FP64 profile and kernel weights (a test and diagnostic switch)."""
+    N.check(N.lib().fsamd_set_fp64(1 if on else 0))
 
 
 def set_stats(enable: bool, timing_only: bool = False):

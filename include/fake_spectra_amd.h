@@ -99,6 +99,12 @@ typedef struct fsamd_line_grad {
 } fsamd_line_grad;
 
 int fsamd_plan_keep_windows(fsamd_plan *plan, int enable);
+/* Counter bumped each time the plan's recorded windows change (a recording
+ * fsamd_tau call, or recording switched off).  A caller that runs the adjoint
+ * later (torch.autograd) saves it after the forward and checks it before the
+ * adjoint, so that windows overwritten by another call in between are caught
+ * instead of silently used.  No counterpart in the reference. */
+int fsamd_plan_window_generation(const fsamd_plan *plan, long long *gen);
 int fsamd_tau_adjoint(fsamd_plan *plan, int nbins, int kernel, double velfac, const float *pos,
                       const float *vel, const float *temp, const float *h,
                       const fsamd_line_grad *lines, int nlines, double *grad_temp,
@@ -160,6 +166,14 @@ int fsamd_tau_autotune(fsamd_plan *plan, int nbins, int kernel, double velfac, d
  * inside a replay: it sets a sticky flag that fsamd_plan_overflow reads
  * (synchronizing the stream); if set, rerun the call uncaptured. */
 int fsamd_plan_overflow(fsamd_plan *plan, int *flag, void *stream);
+
+/* Test and diagnostic switch with no counterpart in the reference: with
+ * on != 0, fsamd_tau and fsamd_tau_adjoint calls made afterwards on this host
+ * thread evaluate every pair with the FP64 Faddeeva function and FP64 kernel
+ * weights (the path the FP32 kernel otherwise keeps for pairs with a damping
+ * parameter above 0.04), e.g. for finite-difference gradients free of FP32
+ * rounding.  Much slower; default off. */
+int fsamd_set_fp64(int on);
 
 /* Tuning knob with no counterpart in the reference: pixels walked per lane
  * group of the tau kernel (16, 32 or 64; a 64-lane wavefront walks 64/g

@@ -111,6 +111,23 @@ class _Lib:
         assert m2 == m
         return off, parts[:m], dr2[:m]
 
+    def table_seconds(self, box, pos, h, axis, cofm):
+        """Wall time of building the neighbour lists alone (IndexTable +
+        get_near_particles, counted but not copied out), one call."""
+        import time
+        pos = _c(pos, np.float32).reshape(-1, 3)
+        axis = _c(axis, np.int32)
+        cofm = _c(cofm, np.float64).reshape(-1, 3)
+        L = axis.shape[0]
+        off = np.zeros(L + 1, dtype=np.int64)
+        dummy_i = np.zeros(1, dtype=np.int32)
+        dummy_d = np.zeros(1, dtype=np.float64)
+        hh = _c(h, np.float32)
+        t0 = time.perf_counter()
+        m = self._f("near_particles")(box, pos.ravel(), hh, pos.shape[0], axis, cofm.ravel(), L,
+                                      off, dummy_i, dummy_d, 0)
+        return time.perf_counter() - t0, int(m)
+
     def near_lines(self, box, pos, h, axis, cofm):
         pos = _c(pos, np.float32).reshape(-1, 3)
         axis = _c(axis, np.int32)

@@ -15,6 +15,7 @@ import math
 
 import numpy as np
 import torch
+import torch.utils.checkpoint
 
 SQPI = math.sqrt(math.pi)
 TWO_SQPI = 2.0 / SQPI
@@ -86,7 +87,23 @@ def sph_cubic(q):
     return np.where(q >= 1, 0.0, r)
 
 
-def tau_torch(pairs, line, nbins, box, velfac, kernel, dens, temp, vlos, tautail=1e-7):
+def sph_quintic(q):
+    """singleabs.h:31-42: 9/(40 pi) sum_k c_k max(0, r_k - 3q)^5, (r, c) = (3, 1), (2, -6), (1, 15)."""
+    q = np.asarray(q, dtype=np.float64)
+    t = lambda r: np.maximum(r - 3.0 * q, 0.0) ** 5
+    return np.where(q >= 1, 0.0, 9.0 / (40.0 * math.pi) * (t(3.0) - 6.0 * t(2.0) + 15.0 * t(1.0)))
+
+
+def kernel_weight(kernel, q):
+    """Weight of a kernel sample in tau_kern_inner (singleabs.h:158-163)."""
+    if kernel == 1:
+        return sph_cubic(q)
+    if kernel == 3:
+        return sph_quintic(q)
+    return np.full_like(np.asarray(q, dtype=np.float64), 3.0 / 4.0 / math.pi)
+
+
+def tau_torch(pairs, line, nbins, box, velfac, kernel, dens, temp, vlos, tautail=1e-7, chunk=None):
     """Optical depth [nlos, nbins] for the listed pairs.
 
     pairs: dict of numpy arrays over pairs: los (sightline), part (particle),
@@ -94,7 +111,10 @@ def tau_torch(pairs, line, nbins, box, velfac, kernel, dens, temp, vlos, tautail
     line: (lambda_cm, gamma, fosc, amumass).
     dens, temp, vlos: float64 torch tensors over particles (leaves for grads);
     vlos is the velocity component along each pair's sightline, indexed by
-    pairs['vidx'] into a flat [N*3] tensor."""
+    pairs['vidx'] into a flat [N*3] tensor.
+    chunk: if set, pairs are processed in chunks of this many under
+    torch.utils.checkpoint (recomputed in backward), which bounds the memory
+    of the autograd graph for a few thousand pairs."""
     lam, gamma, fosc, amu = line
     sigma_a = math.sqrt(3.0 * math.pi * SIGMA_T / 8.0) * lam * fosc
     bfac = math.sqrt(2.0 * BOLTZMANN / (amu * PROTONMASS)) / 1e5
@@ -118,7 +138,7 @@ def tau_torch(pairs, line, nbins, box, velfac, kernel, dens, temp, vlos, tautail
     ii = np.arange(1, 8)
     vv = ii[None, :] * dvk[:, None] - vhigh[:, None]                       # [P,7]
     q = np.sqrt(vdr2[:, None] + vv * vv) / vsm[:, None]
-    W = sph_cubic(q) if kernel == 1 else np.full_like(q, 3.0 / 4.0 / math.pi)
+    W = kernel_weight(kernel, q)
     aa = voigt_fac / b
     amp = sigma_a / SQPI * (LIGHT / 1e5 / b)
     zmax = torch.floor(vel.detach() / bintov).long()
@@ -132,10 +152,8 @@ def tau_torch(pairs, line, nbins, box, velfac, kernel, dens, temp, vlos, tautail
     tau = torch.zeros(pairs["nlos"] * nbins, dtype=torch.float64, device=dv)
     vv_t = torch.as_tensor(vv, device=dv)
     W_t = torch.as_tensor(W, device=dv)
-    for npv in np.unique(npts):
-        sel = np.nonzero((npts == npv) & keep_pair)[0]
-        if sel.size == 0:
-            continue
+
+    def contribution(npv, sel, dens):
         st = torch.as_tensor(sel, device=dv)
         vlow = z[st].double() * bintov - vel[st][:, None]                   # [p, 2h]
         if npv == 1:
@@ -163,7 +181,19 @@ def tau_torch(pairs, line, nbins, box, velfac, kernel, dens, temp, vlos, tautail
         j = torch.remainder(z[st], nbins)
         rows = torch.as_tensor(pairs["los"][sel], device=dv)[:, None].expand_as(j)
         flat = (rows * nbins + j)[keep]
-        tau = tau.index_add(0, flat, t[keep])
+        return torch.zeros(pairs["nlos"] * nbins, dtype=torch.float64, device=dv).index_add(0, flat, t[keep])
+
+    for npv in np.unique(npts):
+        sel_all = np.nonzero((npts == npv) & keep_pair)[0]
+        if sel_all.size == 0:
+            continue
+        step = sel_all.size if chunk is None else chunk
+        for c0 in range(0, sel_all.size, step):
+            sel = sel_all[c0:c0 + step]
+            if chunk is None:
+                tau = tau + contribution(npv, sel, dens)
+            else:
+                tau = tau + torch.utils.checkpoint.checkpoint(contribution, npv, sel, dens, use_reentrant=False)
     return tau.reshape(pairs["nlos"], nbins)
 
 
