@@ -163,18 +163,16 @@ def parity_rows(w, gpu_rows, ref_rows):
 
 def near_lines_leg(w, pos, hh, ax, cofm, with_cpu, budget_s):
     """_near_lines (py_module.cpp:25-99), the prefilter timed on its own: the
-    GPU on the full, device-resident particle set (binning + compaction, median
-    of 3 calls) and, on the host, the reference on every k-th particle with all
+    GPU on the full, device-resident particle set (sightline grid, one
+    early-exit acceptance test per particle and a compaction; median of 3
+    calls) and, on the host, the reference on every k-th particle with all
     sightlines (the sample grown to about budget_s), both per particle."""
     import numpy as np
     import torch
     from fake_spectra_amd import spectra_priv as SP
 
     def run():
-        plan = SP.Plan(w.box, pos, hh, ax, cofm)
-        ids = plan.near_lines()
-        plan.close()
-        return ids
+        return SP.near_lines(w.box, pos, hh, ax, cofm)
 
     ids = run()
     torch.cuda.synchronize()

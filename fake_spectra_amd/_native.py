@@ -27,7 +27,8 @@ EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan
            "fsamd_voigt_peak", "fsamd_set_group_size", "fsamd_plan_keep_windows", "fsamd_tau_adjoint",
            "fsamd_tau_autotune", "fsamd_plan_overflow", "fsamd_mean_flux_scale", "fsamd_res_corr",
            "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms", "fsamd_set_block_waves",
-           "fsamd_voigt_eval", "fsamd_plan_window_generation", "fsamd_set_fp64")
+           "fsamd_voigt_eval", "fsamd_plan_window_generation", "fsamd_set_fp64",
+           "fsamd_near_lines", "fsamd_plan_assign_cells")
 
 
 class FsamdLine(C.Structure):
@@ -61,6 +62,10 @@ def _declare(L):
     L.fsamd_plan_pairs.restype = i
     L.fsamd_plan_near_lines.argtypes = [vp, vp, C.POINTER(ll), vp]
     L.fsamd_plan_near_lines.restype = i
+    L.fsamd_plan_assign_cells.argtypes = [vp, vp, vp]
+    L.fsamd_plan_assign_cells.restype = i
+    L.fsamd_near_lines.argtypes = [d, vp, vp, ll, vp, vp, i, vp, C.POINTER(ll), vp]
+    L.fsamd_near_lines.restype = i
     L.fsamd_tau.argtypes = [vp, i, i, d, d, vp, vp, vp, vp, C.POINTER(FsamdLine), i, i, vp]
     L.fsamd_tau.restype = i
     L.fsamd_colden.argtypes = [vp, i, i, d, vp, vp, C.POINTER(FsamdLine), i, vp]

@@ -365,7 +365,8 @@ __device__ inline void particle_plane(int ax, const float *p, float &first, floa
     }
 }
 
-// Visit every sightline accepted for particle p; f(sightline, dr2).
+// Visit every sightline accepted for particle p; f(sightline, dr2) returns
+// false to stop the visit early.
 template <class F>
 __device__ inline void visit_accepted(const BinArgs &A, const float *p, float hh, F &&f)
 {
@@ -408,7 +409,8 @@ __device__ inline void visit_accepted(const BinArgs &A, const float *p, float hh
                 for (int k = A.grid.cell_start[cell]; k < e; ++k) {
                     const double2 kl = A.grid.cell_kl[k];
                     double dr2;
-                    if (los_accept(first, second, hh, kl.x, kl.y, A.box, dr2)) f(A.grid.cell_ids[k], dr2);
+                    if (los_accept(first, second, hh, kl.x, kl.y, A.box, dr2) && !f(A.grid.cell_ids[k], dr2))
+                        return;
                 }
             }
         }
@@ -421,7 +423,10 @@ __global__ void bin_count_kernel(BinArgs A, int *cnt)
          i += (long long)gridDim.x * blockDim.x) {
         const float p[3] = {A.pos[3 * i], A.pos[3 * i + 1], A.pos[3 * i + 2]};
         int c = 0;
-        visit_accepted(A, p, A.h[i], [&](int, double) { ++c; });
+        visit_accepted(A, p, A.h[i], [&](int, double) {
+            ++c;
+            return true;
+        });
         cnt[i] = c;
     }
 }
@@ -436,7 +441,24 @@ __global__ void bin_emit_kernel(BinArgs A, const long long *poff, int *keys, int
             keys[at] = s;
             vals[at] = (int)i;
             ++at;
+            return true;
         });
+    }
+}
+
+// flag[i] = 1 if particle i reaches at least one sightline: the visit stops at
+// the first accepted sightline (the _near_lines test, py_module.cpp:58-76).
+__global__ void bin_any_kernel(BinArgs A, int *flag)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.npart;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float p[3] = {A.pos[3 * i], A.pos[3 * i + 1], A.pos[3 * i + 2]};
+        int any = 0;
+        visit_accepted(A, p, A.h[i], [&](int, double) {
+            any = 1;
+            return false;
+        });
+        flag[i] = any;
     }
 }
 
@@ -549,6 +571,240 @@ __global__ void near_scatter_kernel(const long long *poff, const long long *fsca
         if (poff[i + 1] > poff[i]) out[fscan[i]] = (int)i;
 }
 
+__global__ void flag_scatter_kernel(const int *flag, const long long *fscan, long long n, int *out)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        if (flag[i]) out[fscan[i]] = (int)i;
+}
+
+// ---------------------------------------------------------------------------
+// Voronoi mesh (kernel 2): IndexTable::assign_cells (index_table.cpp:152-223).
+// Each sightline is cut into npts = int(box/RESO) points at (i + 1/2) reso;
+// point i goes to the nearest particle of the sightline's neighbour list
+// (periodic distance, the first of the ascending row on ties), and a
+// particle's segment runs from its first to its last point, +- reso/2.  The
+// reference walks the points in order; here the same result comes from
+// three parallel passes:
+//   A: over the points past the middle of the box (xp > box/2 + reso/2), the
+//      first one owned by the owner of point 0 (the periodic wrap, where the
+//      reference restarts that particle's segment and stops the walk);
+//   B: over the points before it, each particle's first and last point and
+//      count (atomics);
+//   C: per particle, the segment in FP32 with the reference's roundings; a
+//      particle whose points are not contiguous marks the sightline in
+//      error, where the reference exits (index_table.cpp:199-206).
+
+struct VorArgs {
+    double box, reso;
+    int npts;                  // int(box / RESO)
+    int imin;                  // first point with xp > box/2 + reso/2
+    const float *pos;
+    const int *axis;
+    const double *cofm;
+    const long long *line_off;
+    const int *pair_part;
+    int nlos;
+    int *nn0;                  // [L] owner of point 0 (row index)
+    int *istar;                // [L] first point >= imin owned by nn0; npts if none
+    int *first, *last, *count; // [M] per pair
+    int *err;                  // [L]
+    float2 *seg;               // [M] segment ends
+};
+
+constexpr double kReso = 0.1;     // index_table.h:8
+constexpr int kVorThreads = 256;
+constexpr int kVorTile = 1024;   // row particles staged in LDS at a time
+
+// Owner of the points handled by this block's threads (xp per thread): the
+// row's particles are staged in LDS (coordinates along the sightline and the
+// two transverse ones), every thread scans them in row order.
+__device__ inline void vor_owners(const VorArgs &V, int s, double xp, bool active, int &owner)
+{
+    __shared__ float3 tile[kVorTile];
+    const int ax = V.axis[s];
+    const double *c = &V.cofm[3 * (size_t)s];
+    const double yp = c[ax % 3], zp = c[(ax + 1) % 3];
+    const long long p0 = V.line_off[s], n = V.line_off[s + 1] - p0;
+    const double half = V.box / 2.;
+    double best = V.box, best2 = V.box * V.box;
+    int bi = 0;
+    for (long long t0 = 0; t0 < n; t0 += kVorTile) {
+        const int tn = (int)min((long long)kVorTile, n - t0);
+        __syncthreads();
+        for (int j = threadIdx.x; j < tn; j += kVorThreads) {
+            const float *pp = &V.pos[3 * (size_t)V.pair_part[p0 + t0 + j]];
+            tile[j] = make_float3(pp[ax - 1], pp[ax % 3], pp[(ax + 1) % 3]);
+        }
+        __syncthreads();
+        if (active) {
+            for (int j = 0; j < tn; ++j) {
+                const float3 q = tile[j];
+                double dx = fabs((double)q.x - xp);
+                if (dx > half) dx = V.box - dx;
+                double dy = fabs((double)q.y - yp);
+                if (dy > half) dy = V.box - dy;
+                double dz = fabs((double)q.z - zp);
+                if (dz > half) dz = V.box - dz;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                // sqrt is monotonic: only a smaller d2 can give a smaller distance
+                if (d2 < best2) {
+                    const double d = sqrt(d2);
+                    if (d < best) {
+                        best = d;
+                        best2 = d2;
+                        bi = (int)(t0 + j);
+                    }
+                }
+            }
+        }
+    }
+    owner = bi;
+}
+
+// Owner of point 0 of each sightline (one block per sightline).
+__global__ __launch_bounds__(kVorThreads) void vor_first_kernel(VorArgs V)
+{
+    const int s = blockIdx.x;
+    if (V.line_off[s + 1] == V.line_off[s]) {
+        if (threadIdx.x == 0) V.nn0[s] = -1;
+        return;
+    }
+    int owner = 0;
+    vor_owners(V, s, 0.5 * V.reso, threadIdx.x == 0, owner);
+    if (threadIdx.x == 0) {
+        V.nn0[s] = owner;
+        V.istar[s] = V.npts;
+    }
+}
+
+// Pass A: blocks over (chunk of points from imin, sightline).
+__global__ __launch_bounds__(kVorThreads) void vor_wrap_kernel(VorArgs V)
+{
+    __shared__ int s_skip;
+    const int s = blockIdx.y;
+    const int i0 = V.imin + blockIdx.x * kVorThreads;
+    if (V.nn0[s] < 0 || i0 >= V.npts) return;
+    // an earlier point already wraps: read once for the whole block
+    if (threadIdx.x == 0) s_skip = __atomic_load_n(&V.istar[s], __ATOMIC_RELAXED) < i0;
+    __syncthreads();
+    if (s_skip) return;
+    const int i = i0 + threadIdx.x;
+    int owner = 0;
+    vor_owners(V, s, (i + 0.5) * V.reso, i < V.npts, owner);
+    if (i < V.npts && owner == V.nn0[s]) atomicMin(&V.istar[s], i);
+}
+
+// Pass B: blocks over (chunk of points from 0, sightline), points before istar.
+__global__ __launch_bounds__(kVorThreads) void vor_cells_kernel(VorArgs V)
+{
+    const int s = blockIdx.y;
+    const int i0 = blockIdx.x * kVorThreads;
+    if (V.nn0[s] < 0 || i0 >= V.istar[s]) return;
+    const int i = i0 + threadIdx.x;
+    const bool active = i < V.istar[s];
+    int owner = 0;
+    vor_owners(V, s, (i + 0.5) * V.reso, active, owner);
+    if (active) {
+        const long long k = V.line_off[s] + owner;
+        atomicMin(&V.first[k], i);
+        atomicMax(&V.last[k], i);
+        atomicAdd(&V.count[k], 1);
+    }
+}
+
+// Pass C: one block per sightline, threads over its particles.
+__global__ __launch_bounds__(kVorThreads) void vor_segments_kernel(VorArgs V)
+{
+    const int s = blockIdx.x;
+    const long long p0 = V.line_off[s], n = V.line_off[s + 1] - p0;
+    const double reso = V.reso, box = V.box;
+    for (long long j = threadIdx.x; j < n; j += kVorThreads) {
+        const long long k = p0 + j;
+        const int c = V.count[k];
+        float lo = (float)(3 * box), hi = (float)(3 * box);
+        if (c > 0) {
+            const int f = V.first[k], l = V.last[k];
+            if (c != l - f + 1) atomicOr(&V.err[s], 1);   // two separate runs: the reference exits
+            lo = (float)((f + 0.5) * reso);
+            hi = (float)((l + 0.5) * reso);
+            if (j == V.nn0[s] && V.istar[s] < V.npts) {
+                // the wrap (index_table.cpp:187-196): the segment restarts at
+                // istar and its end moves on by one box length
+                lo = (float)((V.istar[s] + 0.5) * reso);
+                hi = (float)((double)hi + box);
+            }
+        }
+        // index_table.cpp:218-221
+        V.seg[k] = make_float2((float)((double)lo - 0.5 * reso), (float)((double)hi + 0.5 * reso));
+    }
+}
+
+int plan_assign_cells(fsamd_plan *P, const float *pos, hipStream_t s)
+{
+    int rc;
+    const long long M = P->npairs;
+    const int L = P->nlos;
+    if ((rc = P->seg.alloc(sizeof(float2) * (size_t)std::max<long long>(M, 1), P->arena))) return rc;
+    P->seg_ready = false;
+    if (L == 0 || M == 0) {
+        P->seg_ready = true;
+        return FSAMD_OK;
+    }
+    const int npts = (int)(P->box / kReso);
+    if (npts < 1) return fail(FSAMD_ERR_ARG, "the box is shorter than the Voronoi grid spacing (0.1)");
+    VorArgs V;
+    V.box = P->box;
+    V.reso = P->box / npts;
+    V.npts = npts;
+    // first point past the middle of the box (the wrap test of index_table.cpp:187)
+    int imin = 0;
+    while (imin < npts && !((imin + 0.5) * V.reso > P->box / 2. + 0.5 * V.reso)) ++imin;
+    V.imin = imin;
+    V.pos = pos;
+    V.axis = P->axis.as<int>();
+    V.cofm = P->cofm.as<double>();
+    V.line_off = P->line_off.as<long long>();
+    V.pair_part = P->pair_part.as<int>();
+    V.nlos = L;
+    DevBuf nn0, istar, first, last, count, err;
+    if ((rc = nn0.alloc(sizeof(int) * (size_t)L, P->arena)) || (rc = istar.alloc(sizeof(int) * (size_t)L, P->arena)) ||
+        (rc = err.alloc(sizeof(int) * (size_t)L, P->arena)) ||
+        (rc = first.alloc(sizeof(int) * (size_t)M, P->arena)) || (rc = last.alloc(sizeof(int) * (size_t)M, P->arena)) ||
+        (rc = count.alloc(sizeof(int) * (size_t)M, P->arena)))
+        return rc;
+    V.nn0 = nn0.as<int>();
+    V.istar = istar.as<int>();
+    V.err = err.as<int>();
+    V.first = first.as<int>();
+    V.last = last.as<int>();
+    V.count = count.as<int>();
+    V.seg = P->seg.as<float2>();
+    if ((rc = launch_fill_i32(V.err, 0, L, s)) || (rc = launch_fill_i32(V.first, INT32_MAX, M, s)) ||
+        (rc = launch_fill_i32(V.last, -1, M, s)) || (rc = launch_fill_i32(V.count, 0, M, s)))
+        return rc;
+    vor_first_kernel<<<L, kVorThreads, 0, s>>>(V);
+    FSAMD_HIP(hipGetLastError());
+    const int chunks_a = (npts - imin + kVorThreads - 1) / kVorThreads;
+    if (chunks_a > 0) {
+        vor_wrap_kernel<<<dim3(chunks_a, L), kVorThreads, 0, s>>>(V);
+        FSAMD_HIP(hipGetLastError());
+    }
+    vor_cells_kernel<<<dim3((npts + kVorThreads - 1) / kVorThreads, L), kVorThreads, 0, s>>>(V);
+    FSAMD_HIP(hipGetLastError());
+    vor_segments_kernel<<<L, kVorThreads, 0, s>>>(V);
+    FSAMD_HIP(hipGetLastError());
+    std::vector<int> herr(L);
+    FSAMD_HIP(hipMemcpyAsync(herr.data(), V.err, sizeof(int) * L, hipMemcpyDeviceToHost, s));
+    FSAMD_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < L; ++i)
+        if (herr[i])
+            return fail(FSAMD_ERR_ARG, "Voronoi cell assignment: a particle owns two separate runs of sightline " +
+                                           std::to_string(i) + " (the reference exits here, index_table.cpp:199-206)");
+    P->seg_ready = true;
+    return FSAMD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 
@@ -556,15 +812,29 @@ DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release()
 {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !borrowed) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+    borrowed = false;
 }
 
-int DevBuf::alloc(size_t n)
+int DevBuf::alloc(size_t n, Arena *arena)
 {
     release();
     if (n == 0) n = 16;
+    if (arena) {
+        const size_t at = (arena->used + 255) & ~(size_t)255;
+        if (at > arena->cap || n > arena->cap - at)
+            return fail(FSAMD_ERR_UNSUPPORTED, "this call needs more device memory than the " +
+                                                   std::to_string(arena->cap >> 20) +
+                                                   " MiB scratch of the allocation-free entry; "
+                                                   "use the allocating entry point");
+        ptr = arena->base + at;
+        bytes = n;
+        borrowed = true;
+        arena->used = at + n;
+        return FSAMD_OK;
+    }
     hipError_t e = hipMalloc(&ptr, n);
     if (e != hipSuccess) {
         ptr = nullptr;
@@ -582,22 +852,23 @@ static int choose_cells(int n)
     return std::max(1, std::min(c, 4096));
 }
 
-int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long long npart,
-               const int *axis, const double *cofm, int nlos, hipStream_t s)
+// The sightline grid of a plan (IndexTable ctor, index_table.cpp:7-18):
+// copies of axis/cofm, per-axis cell counts, cell starts, sightline ids and
+// (key, second) coordinates in cell order.
+static int grid_build(fsamd_plan *P, double box, const int *axis, const double *cofm, int nlos, hipStream_t s)
 {
     int rc;
     P->box = box;
-    P->npart = npart;
     P->nlos = nlos;
-    if ((rc = P->axis.alloc(sizeof(int) * (size_t)nlos))) return rc;
-    if ((rc = P->cofm.alloc(sizeof(double) * 3 * (size_t)nlos))) return rc;
+    if ((rc = P->axis.alloc(sizeof(int) * (size_t)nlos, P->arena))) return rc;
+    if ((rc = P->cofm.alloc(sizeof(double) * 3 * (size_t)nlos, P->arena))) return rc;
     if (nlos > 0) {
         FSAMD_HIP(hipMemcpyAsync(P->axis.ptr, axis, sizeof(int) * nlos, hipMemcpyDeviceToDevice, s));
         FSAMD_HIP(hipMemcpyAsync(P->cofm.ptr, cofm, sizeof(double) * 3 * nlos, hipMemcpyDeviceToDevice, s));
     }
     // --- sightline grid
     DevBuf counts3;
-    if ((rc = counts3.alloc(sizeof(int) * 4))) return rc;
+    if ((rc = counts3.alloc(sizeof(int) * 4, P->arena))) return rc;
     FSAMD_HIP(hipMemsetAsync(counts3.ptr, 0, sizeof(int) * 4, s));
     if (nlos > 0) {
         los_axis_count_kernel<<<(nlos + 255) / 256, 256, 0, s>>>(P->axis.as<int>(), nlos, counts3.as<int>());
@@ -618,13 +889,13 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     }
     P->ncells = ncells;
     DevBuf cell_of, cell_count, cell_cursor, scan_ws, cell_start64;
-    if ((rc = cell_of.alloc(sizeof(int) * (size_t)std::max(nlos, 1)))) return rc;
-    if ((rc = cell_count.alloc(sizeof(int) * (size_t)(ncells + 1)))) return rc;
-    if ((rc = cell_start64.alloc(sizeof(long long) * (size_t)(ncells + 1)))) return rc;
-    if ((rc = P->cell_start.alloc(sizeof(int) * (size_t)(ncells + 1)))) return rc;
-    if ((rc = P->cell_ids.alloc(sizeof(int) * (size_t)std::max(nlos, 1)))) return rc;
-    if ((rc = P->cell_kl.alloc(sizeof(double2) * (size_t)std::max(nlos, 1)))) return rc;
-    if ((rc = scan_ws.alloc(exclusive_scan_ws_bytes(std::max<long long>(npart, ncells) + 1)))) return rc;
+    if ((rc = cell_of.alloc(sizeof(int) * (size_t)std::max(nlos, 1), P->arena))) return rc;
+    if ((rc = cell_count.alloc(sizeof(int) * (size_t)(ncells + 1), P->arena))) return rc;
+    if ((rc = cell_start64.alloc(sizeof(long long) * (size_t)(ncells + 1), P->arena))) return rc;
+    if ((rc = P->cell_start.alloc(sizeof(int) * (size_t)(ncells + 1), P->arena))) return rc;
+    if ((rc = P->cell_ids.alloc(sizeof(int) * (size_t)std::max(nlos, 1), P->arena))) return rc;
+    if ((rc = P->cell_kl.alloc(sizeof(double2) * (size_t)std::max(nlos, 1), P->arena))) return rc;
+    if ((rc = scan_ws.alloc(exclusive_scan_ws_bytes((long long)ncells + 1), P->arena))) return rc;
     if ((rc = launch_fill_i32(cell_count.as<int>(), 0, ncells + 1, s))) return rc;
     if (nlos > 0) {
         los_cell_kernel<<<(nlos + 255) / 256, 256, 0, s>>>(P->axis.as<int>(), P->cofm.as<double>(), nlos, box, g,
@@ -637,7 +908,7 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     i64_to_i32_kernel<<<grid_for(ncells + 1, 256), 256, 0, s>>>(cell_start64.as<long long>(),
                                                                 P->cell_start.as<int>(), ncells + 1);
     FSAMD_HIP(hipGetLastError());
-    if ((rc = cell_cursor.alloc(sizeof(int) * (size_t)(ncells + 1)))) return rc;
+    if ((rc = cell_cursor.alloc(sizeof(int) * (size_t)(ncells + 1), P->arena))) return rc;
     FSAMD_HIP(hipMemcpyAsync(cell_cursor.ptr, P->cell_start.ptr, sizeof(int) * (ncells + 1),
                              hipMemcpyDeviceToDevice, s));
     if (nlos > 0) {
@@ -649,6 +920,18 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     g.cell_start = P->cell_start.as<int>();
     g.cell_ids = P->cell_ids.as<int>();
     g.cell_kl = P->cell_kl.as<double2>();
+    return FSAMD_OK;
+}
+
+int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long long npart,
+               const int *axis, const double *cofm, int nlos, hipStream_t s)
+{
+    int rc;
+    P->npart = npart;
+    if ((rc = grid_build(P, box, axis, cofm, nlos, s))) return rc;
+    const LosGrid &g = P->grid;
+    DevBuf scan_ws;
+    if ((rc = scan_ws.alloc(exclusive_scan_ws_bytes(npart + 1), P->arena))) return rc;
 
     // --- count, scan, emit
     BinArgs A;
@@ -658,8 +941,8 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     A.npart = npart;
     A.grid = g;
     DevBuf cnt;
-    if ((rc = cnt.alloc(sizeof(int) * (size_t)std::max<long long>(npart, 1)))) return rc;
-    if ((rc = P->poff.alloc(sizeof(long long) * (size_t)(npart + 1)))) return rc;
+    if ((rc = cnt.alloc(sizeof(int) * (size_t)std::max<long long>(npart, 1), P->arena))) return rc;
+    if ((rc = P->poff.alloc(sizeof(long long) * (size_t)(npart + 1), P->arena))) return rc;
     if (npart > 0) {
         bin_count_kernel<<<grid_for(npart, 256, 1 << 20), 256, 0, s>>>(A, cnt.as<int>());
         FSAMD_HIP(hipGetLastError());
@@ -674,7 +957,7 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     P->npairs = M;
     DevBuf k0, v0, k1, v1;
     const size_t mbytes = sizeof(int) * (size_t)std::max<long long>(M, 1);
-    if ((rc = k0.alloc(mbytes)) || (rc = v0.alloc(mbytes)) || (rc = k1.alloc(mbytes)) || (rc = v1.alloc(mbytes)))
+    if ((rc = k0.alloc(mbytes, P->arena)) || (rc = v0.alloc(mbytes, P->arena)) || (rc = k1.alloc(mbytes, P->arena)) || (rc = v1.alloc(mbytes, P->arena)))
         return rc;
     if (npart > 0 && M > 0) {
         bin_emit_kernel<<<grid_for(npart, 256, 1 << 20), 256, 0, s>>>(A, P->poff.as<long long>(), k0.as<int>(),
@@ -690,8 +973,8 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     if (M > 0) {
         const long long ntiles = (M + kRsTile - 1) / kRsTile;
         DevBuf table, tws;
-        if ((rc = table.alloc(sizeof(long long) * (size_t)((1 << bits_per) * ntiles + 1)))) return rc;
-        if ((rc = tws.alloc(exclusive_scan_ws_bytes((1 << bits_per) * ntiles + 1)))) return rc;
+        if ((rc = table.alloc(sizeof(long long) * (size_t)((1 << bits_per) * ntiles + 1), P->arena))) return rc;
+        if ((rc = tws.alloc(exclusive_scan_ws_bytes((1 << bits_per) * ntiles + 1), P->arena))) return rc;
         for (int ps = 0; ps < passes; ++ps) {
             const int shift = ps * bits_per;
             const int bits = std::min(bits_per, total_bits - shift);
@@ -711,7 +994,7 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     }
     // (kin, vin) hold the sorted sightline ids and particle ids
     P->pair_part.take(*vin);
-    if ((rc = P->line_off.alloc(sizeof(long long) * (size_t)(nlos + 1)))) return rc;
+    if ((rc = P->line_off.alloc(sizeof(long long) * (size_t)(nlos + 1), P->arena))) return rc;
     row_offsets_kernel<<<(nlos + 1 + 255) / 256, 256, 0, s>>>(kin->as<int>(), M, nlos,
                                                              P->line_off.as<long long>());
     FSAMD_HIP(hipGetLastError());
@@ -728,11 +1011,43 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     });
     P->max_row = 0;
     for (int i = 0; i < nlos; ++i) P->max_row = std::max(P->max_row, off[i + 1] - off[i]);
-    if ((rc = P->order.alloc(sizeof(int) * (size_t)std::max(nlos, 1)))) return rc;
+    if ((rc = P->order.alloc(sizeof(int) * (size_t)std::max(nlos, 1), P->arena))) return rc;
     if (nlos > 0)
         FSAMD_HIP(hipMemcpy(P->order.ptr, order.data(), sizeof(int) * nlos, hipMemcpyHostToDevice));
     P->order_host = std::move(order);
     P->off_host = std::move(off);
+    return FSAMD_OK;
+}
+
+int near_lines_direct(double box, const float *pos, const float *h, long long npart, const int *axis,
+                      const double *cofm, int nlos, int *out, long long *n_out, hipStream_t s)
+{
+    int rc;
+    *n_out = 0;
+    if (npart == 0 || nlos == 0) return FSAMD_OK;
+    fsamd_plan G;   // holds the grid's buffers for this call
+    G.npart = npart;
+    if ((rc = grid_build(&G, box, axis, cofm, nlos, s))) return rc;
+    BinArgs A;
+    A.box = box;
+    A.pos = pos;
+    A.h = h;
+    A.npart = npart;
+    A.grid = G.grid;
+    DevBuf flag, fscan, ws;
+    if ((rc = flag.alloc(sizeof(int) * (size_t)npart))) return rc;
+    if ((rc = fscan.alloc(sizeof(long long) * (size_t)(npart + 1)))) return rc;
+    if ((rc = ws.alloc(exclusive_scan_ws_bytes(npart + 1)))) return rc;
+    bin_any_kernel<<<grid_for(npart, 256, 1 << 20), 256, 0, s>>>(A, flag.as<int>());
+    FSAMD_HIP(hipGetLastError());
+    if ((rc = exclusive_scan_i32_to_i64(flag.as<int>(), fscan.as<long long>(), npart, ws.ptr, ws.bytes, s)))
+        return rc;
+    flag_scatter_kernel<<<grid_for(npart, 256), 256, 0, s>>>(flag.as<int>(), fscan.as<long long>(), npart, out);
+    FSAMD_HIP(hipGetLastError());
+    long long cnt = 0;
+    FSAMD_HIP(hipMemcpyAsync(&cnt, fscan.as<long long>() + npart, sizeof(long long), hipMemcpyDeviceToHost, s));
+    FSAMD_HIP(hipStreamSynchronize(s));
+    *n_out = cnt;
     return FSAMD_OK;
 }
 

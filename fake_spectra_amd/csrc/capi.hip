@@ -102,7 +102,7 @@ static int item_set(fsamd_plan *P, int K, long long chunk, fsamd_plan::ItemSet *
         }
     }
     if (items.size() >= (size_t)INT32_MAX) return fail(FSAMD_ERR_ARG, "too many work items");
-    int rc = set->buf.alloc(sizeof(WorkItem) * std::max<size_t>(items.size(), 1));
+    int rc = set->buf.alloc(sizeof(WorkItem) * std::max<size_t>(items.size(), 1), P->arena);
     if (rc) return rc;
     if (!items.empty())
         FSAMD_HIP(hipMemcpy(set->buf.ptr, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
@@ -115,8 +115,8 @@ static int item_set(fsamd_plan *P, int K, long long chunk, fsamd_plan::ItemSet *
 // still address is retired rather than freed.
 static int regrow(fsamd_plan *P, DevBuf &b, size_t bytes)
 {
-    if (P->captured && b.ptr) P->retired.push_back(b.disown());
-    return b.alloc(bytes);
+    if (P->captured && b.ptr && !b.borrowed) P->retired.push_back(b.disown());
+    return b.alloc(bytes, P->arena);
 }
 
 static int grow_fixups(fsamd_plan *P, long long cap, hipStream_t s)
@@ -158,7 +158,7 @@ static int line_table(fsamd_plan *P, const std::vector<LineDesc> &ld, bool captu
     if (!same(e)) {
         FSAMD_HIP(hipStreamSynchronize(s));   // the previous table may still be in use
         if (e.buf.bytes < bytes) {
-            int rc = e.buf.alloc(bytes);
+            int rc = e.buf.alloc(bytes, P->arena);
             if (rc) return rc;
         }
         FSAMD_HIP(hipMemcpy(e.buf.ptr, ld.data(), bytes, hipMemcpyHostToDevice));
@@ -175,8 +175,8 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
 {
     if (!P) return fail(FSAMD_ERR_ARG, "plan is NULL");
     if (nbins < 1) return fail(FSAMD_ERR_ARG, "nbins must be positive");
-    if (kernel != kTopHat && kernel != kCubicSpline && kernel != kSphQuintic)
-        return fail(FSAMD_ERR_UNSUPPORTED, "kernel must be 0 (tophat), 1 (cubic spline) or 3 (quintic spline)");
+    if (kernel == kVoronoiMesh && !P->seg_ready)
+        return fail(FSAMD_ERR_ARG, "kernel 2 (Voronoi mesh) needs fsamd_plan_assign_cells on the plan first");
     if (nlines < 1 || !lines) return fail(FSAMD_ERR_ARG, "at least one line is required");
     if (mode != FSAMD_MODE_BATCHED && mode != FSAMD_MODE_PLAIN) return fail(FSAMD_ERR_ARG, "unknown mode");
     if ((long long)P->nlos * nbins >= (1ll << 62)) return fail(FSAMD_ERR_ARG, "output too large");
@@ -216,7 +216,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         if ((rc = regrow(P, P->queue, sizeof(int) * ngroups))) return rc;
     }
     if (g_stats_on) {
-        if (!P->stats.ptr && (rc = P->stats.alloc(sizeof(long long) * kNStats))) return rc;
+        if (!P->stats.ptr && (rc = P->stats.alloc(sizeof(long long) * kNStats, P->arena))) return rc;
         if ((rc = launch_fill_f64((double *)P->stats.ptr, 0.0, kNStats, s))) return rc;   // all-zero bits
     }
     if (g_timing_on) {
@@ -247,7 +247,7 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         if ((P->fix_cap < want || !P->fix_count.ptr) && capturing)
             return fail(FSAMD_ERR_ARG, "run one uncaptured fsamd_tau call before capturing");
         if (P->fix_cap < want && (rc = grow_fixups(P, want, s))) return rc;
-        if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(unsigned long long) * 2))) return rc;
+        if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(unsigned long long) * 2, P->arena))) return rc;
     }
     if (tau) {
         const size_t rbytes = sizeof(PartRec) * (size_t)std::max<long long>(P->npart, 1);
@@ -279,15 +279,17 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
         A.cofm = P->cofm.as<double>();
         A.line_off = P->line_off.as<long long>();
         A.pair_part = P->pair_part.as<int>();
+        A.seg = kernel == kVoronoiMesh ? P->seg.as<float2>() : nullptr;
         A.nlines = K;
         A.lines = table + k0;
         A.rec = tau ? P->rec.as<PartRec>() : nullptr;
         if (tau && (rc = launch_pack_records(pos, vel, temp, h, A.lines, K, P->npart, P->rec.as<PartRec>(), s)))
             return rc;
         A.queue = P->queue.as<int>() + g;
-        // whole-sightline items: a lane group sets up to kPairQueue pairs at a
+        // whole-sightline items: a wavefront sets up to kWaveClaim pairs at a
         // time; the one-launch-per-sightline mode, whose items are short, one
-        A.claim = plain ? 1 : kPairQueue;
+        // per lane group
+        A.claim = plain ? 64 / gsize : kWaveClaim;
         A.stats = g_stats_on ? P->stats.as<long long>() : nullptr;
         A.npairs = P->npairs;
         A.win = rec ? P->win.as<int>() + (size_t)k0 * 2 * P->npairs : nullptr;
@@ -368,6 +370,17 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
     return FSAMD_OK;
 }
 
+// Arguments of a plan, checked before anything runs (fsamd_plan_create).
+static int check_plan_args(double box, const float *pos, const float *h, long long npart, const int *axis,
+                           const double *cofm, int nlos)
+{
+    if (npart < 0 || nlos < 0) return fail(FSAMD_ERR_ARG, "negative size");
+    if (npart >= (1ll << 31)) return fail(FSAMD_ERR_ARG, "particle indices must fit in int (N < 2^31)");
+    if (!(box > 0) || !std::isfinite(box)) return fail(FSAMD_ERR_ARG, "box must be positive and finite");
+    if ((npart > 0 && (!pos || !h)) || (nlos > 0 && (!axis || !cofm))) return fail(FSAMD_ERR_ARG, "NULL array");
+    return FSAMD_OK;
+}
+
 }  // namespace fsamd
 
 using namespace fsamd;
@@ -383,11 +396,7 @@ int fsamd_plan_create(double box, const float *pos, const float *h, long long np
 {
     if (!out) return fail(FSAMD_ERR_ARG, "out is NULL");
     *out = nullptr;
-    if (npart < 0 || nlos < 0) return fail(FSAMD_ERR_ARG, "negative size");
-    if (npart >= (1ll << 31)) return fail(FSAMD_ERR_ARG, "particle indices must fit in int (N < 2^31)");
-    if (!(box > 0) || !std::isfinite(box)) return fail(FSAMD_ERR_ARG, "box must be positive and finite");
-    if ((npart > 0 && (!pos || !h)) || (nlos > 0 && (!axis || !cofm)))
-        return fail(FSAMD_ERR_ARG, "NULL array");
+    if (int rc = check_plan_args(box, pos, h, npart, axis, cofm, nlos)) return rc;
     fsamd_plan *P = new fsamd_plan();
     int rc = plan_build(P, box, pos, h, npart, axis, cofm, nlos, (hipStream_t)stream);
     if (rc) {
@@ -415,10 +424,27 @@ int fsamd_plan_pairs(const fsamd_plan *plan, const long long **offsets, const in
     return FSAMD_OK;
 }
 
+int fsamd_plan_assign_cells(fsamd_plan *plan, const float *pos, void *stream)
+{
+    if (!plan) return fail(FSAMD_ERR_ARG, "plan is NULL");
+    if (plan->npart > 0 && !pos) return fail(FSAMD_ERR_ARG, "pos is NULL");
+    return plan_assign_cells(plan, pos, (hipStream_t)stream);
+}
+
 int fsamd_plan_near_lines(const fsamd_plan *plan, int *out, long long *n_out, void *stream)
 {
     if (!plan || !n_out) return fail(FSAMD_ERR_ARG, "NULL argument");
     return plan_near_lines(plan, out, n_out, (hipStream_t)stream);
+}
+
+int fsamd_near_lines(double box, const float *pos, const float *h, long long npart, const int *axis,
+                     const double *cofm, int nlos, int *out, long long *n_out, void *stream)
+{
+    if (!n_out) return fail(FSAMD_ERR_ARG, "n_out is NULL");
+    *n_out = 0;
+    if (int rc = check_plan_args(box, pos, h, npart, axis, cofm, nlos)) return rc;
+    if (npart > 0 && nlos > 0 && !out) return fail(FSAMD_ERR_ARG, "out is NULL");
+    return near_lines_direct(box, pos, h, npart, axis, cofm, nlos, out, n_out, (hipStream_t)stream);
 }
 
 int fsamd_tau(fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail, const float *pos,
@@ -446,6 +472,11 @@ int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double bo
     fsamd_plan *plan = nullptr;
     int rc = fsamd_plan_create(box100, pos, h, npart, axis, cofm, nlos, stream, &plan);
     if (rc) return rc;
+    if (kernel == kVoronoiMesh && (rc = plan_assign_cells(plan, pos, (hipStream_t)stream))) {
+        (void)hipStreamSynchronize((hipStream_t)stream);
+        fsamd_plan_destroy(plan);
+        return rc;
+    }
     fsamd_line l;
     l.lambda_cm = lambda;
     l.gamma = gamma;
@@ -530,7 +561,7 @@ int fsamd_plan_keep_windows(fsamd_plan *plan, int enable)
     plan->keep_windows = enable != 0;
     if (!enable) {
         // a graph captured while recording was on still writes this buffer
-        if (plan->captured && plan->win.ptr)
+        if (plan->captured && plan->win.ptr && !plan->win.borrowed)
             plan->retired.push_back(plan->win.disown());
         else
             plan->win.release();
@@ -554,6 +585,8 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
     hipStream_t s = (hipStream_t)stream;
     if (!P) return fail(FSAMD_ERR_ARG, "plan is NULL");
     if (!lines || nlines < 1 || !grad_temp || !grad_vel) return fail(FSAMD_ERR_ARG, "NULL argument");
+    if (kernel == kVoronoiMesh && !P->seg_ready)
+        return fail(FSAMD_ERR_ARG, "kernel 2 (Voronoi mesh) needs fsamd_plan_assign_cells on the plan first");
     if (!P->keep_windows || P->win_lines != nlines || P->win_nbins != nbins || P->win_kernel != kernel)
         return fail(FSAMD_ERR_ARG, "fsamd_tau_adjoint needs the windows of a matching fsamd_tau call on this plan "
                                    "(fsamd_plan_keep_windows(plan, 1) before the forward)");
@@ -605,6 +638,7 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         A.cofm = P->cofm.as<double>();
         A.line_off = P->line_off.as<long long>();
         A.pair_part = P->pair_part.as<int>();
+        A.seg = kernel == kVoronoiMesh ? P->seg.as<float2>() : nullptr;
         A.nlines = K;
         A.lines = table + k0;
         A.rec = P->rec.as<PartRec>();

@@ -82,12 +82,15 @@ __device__ inline double sph_quintic_kernel(double q)
     return norm * f;
 }
 
-// Kernel weight of tau_kern_inner (singleabs.h:158-163)
+// Kernel weight of tau_kern_inner (singleabs.h:158-163): the cubic or quintic
+// spline, 3/(4 pi) for the tophat, and no weight (1) for any other id, which
+// takes none of the reference's branches there.
 __device__ inline double kernel_weight(int kernel, double q)
 {
     if (kernel == kCubicSpline) return sph_cubic_kernel(q);
     if (kernel == kSphQuintic) return sph_quintic_kernel(q);
-    return 3. / 4. / kPi;
+    if (kernel == kTopHat) return 3. / 4. / kPi;
+    return 1.0;
 }
 
 __device__ inline LineDesc line_of(const DepositArgs &A, int k) { return A.lines[k]; }
@@ -156,12 +159,41 @@ __device__ inline int wrap_pixel(int z, int nbins)
 // values are uniform across the wave, and reading them with broadcast LDS
 // loads keeps them out of the vector registers of the per-pixel loop.
 // One interior kernel sample of a pair (singleabs.h:146-157): its offset
-// vv_i / b and weight K(q_i), read together in the sample loop.
+// vv_i / b as a two-float split (cv_hi + cv_lo, ~48 bits) and its weight
+// K(q_i), read together (one 16-byte LDS read) in the sample loop.
 struct KernelSample {
-    double cv;
+    float cv_hi, cv_lo;
     float w;
     float pad;
 };
+
+__device__ inline void set_sample_offset(KernelSample &k, double cv)
+{
+    k.cv_hi = (float)cv;
+    k.cv_lo = (float)(cv - (double)k.cv_hi);
+}
+
+__device__ inline double sample_offset(const KernelSample &k) { return (double)k.cv_hi + (double)k.cv_lo; }
+
+// The sub-sample velocity in units of b as a two-float split.
+struct Split {
+    float hi, lo;
+};
+
+__device__ inline Split split_s(double s)
+{
+    Split r;
+    r.hi = (float)s;
+    r.lo = (float)(s - (double)r.hi);
+    return r;
+}
+
+// u = (vv_i - vouter)/b in FP32 from the two splits: the high parts cancel
+// exactly where u is small (Sterbenz), so u keeps ~1 ulp relative error.
+__device__ inline float sample_u(const KernelSample &k, Split s)
+{
+    return (k.cv_hi - s.hi) + (k.cv_lo - s.lo);
+}
 
 struct PairTau {
     KernelSample smp[kNGrid - 1];   // kernel sample velocities / b and weights
@@ -206,14 +238,18 @@ __device__ inline double shfl_d(double v, int src)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// sum_i K(q_i) Re w((vv_i - vouter)/b + i a), FP32 profile
+// sum_i K(q_i) Re w((vv_i - vouter)/b + i a), FP32 profile; the sample
+// offsets from the FP64 anchor vouter/b (one split per sum)
 __device__ inline double inner32(const PairTau &P, double vouter, const float *gtab)
 {
-    const double s = vouter * P.inv_b;
+    const Split s = split_s(vouter * P.inv_b);
     const VoigtA va = P.va;
     float acc = 0.0f;
 #pragma unroll 2  // measured 0.7% faster at c1 than 1; a full unroll spills
-    for (int i = 0; i < kNGrid - 1; ++i) acc = __builtin_fmaf(P.smp[i].w, voigt32(va, P.smp[i].cv - s, gtab), acc);
+    for (int i = 0; i < kNGrid - 1; ++i) {
+        const KernelSample k = P.smp[i];
+        acc = __builtin_fmaf(k.w, voigt32(va, sample_u(k, s), gtab), acc);
+    }
     return (double)acc;
 }
 
@@ -318,9 +354,9 @@ __device__ inline void wing_setup(PairTau &P, double bintov, int lane)
         }
     }
     // m'_n = sum_k C(2n,2k) X_k Y_(n-k), X_k by running powers
-    const double x4 = (P.smp[4].cv * inv_u) * (P.smp[4].cv * inv_u);
-    const double x5 = (P.smp[5].cv * inv_u) * (P.smp[5].cv * inv_u);
-    const double x6 = (P.smp[6].cv * inv_u) * (P.smp[6].cv * inv_u);
+    const double c4 = sample_offset(P.smp[4]) * inv_u, c5 = sample_offset(P.smp[5]) * inv_u,
+                 c6 = sample_offset(P.smp[6]) * inv_u;
+    const double x4 = c4 * c4, x5 = c5 * c5, x6 = c6 * c6;
     const double w4 = (double)P.smp[4].w + (double)P.smp[2].w;
     const double w5 = (double)P.smp[5].w + (double)P.smp[1].w;
     const double w6 = (double)P.smp[6].w + (double)P.smp[0].w;
@@ -425,33 +461,51 @@ struct PairConsts {
     int zmax, np, fp64;
 };
 
-// Returns false if the particle does not reach the sightline
-// (absorption.cpp:232).  fp64 routes the pair to the FP64 path.
-__device__ inline bool pair_consts(const DepositArgs &A, int line, const LineDesc &L, const LosInfo &los, int p,
-                                   int max_sums, PairConsts &C)
+// Pair kk (the particle at pair_part[kk]).  Returns false if the particle does
+// not reach the sightline (absorption.cpp:232; on the Voronoi mesh, if it owns
+// no segment of it, absorption.cpp:221-229).  fp64 routes the pair to the
+// FP64 path.
+__device__ inline bool pair_consts(const DepositArgs &A, int line, const LineDesc &L, const LosInfo &los,
+                                   long long kk, int max_sums, PairConsts &C)
 {
+    const int p = A.pair_part[kk];
     const float4 r0 = *(const float4 *)&A.rec[p].x;
     const float4 r1 = *(const float4 *)&A.rec[p].vx;
     const float pp[3] = {r0.x, r0.y, r0.z};
-    const double dr2 = pair_dr2(los, pp, A.box);
-    const float smooth = r0.w;
     const float temp = r1.w;
     const float dens = A.rec[p].dens[line];
     const float ppos = pp[los.ax - 1];
     const float pvel = los.ax == 1 ? r1.x : (los.ax == 2 ? r1.y : r1.z);
     const double b = L.bfac * sqrt((double)temp);
-    const float sm2 = smooth * smooth;
-    if ((double)sm2 - dr2 <= 0) return false;            // absorption.cpp:232
-    const double vel = A.velfac * (double)ppos + (double)pvel;
-    double val1 = A.velfac * dr2;
-    val1 *= A.velfac;
-    const double vdr2 = val1;
-    const double vsmooth = A.velfac * (double)smooth;
+    double pos1 = (double)ppos, vdr2, vsmooth, vhigh;
+    if (A.seg) {
+        // Voronoi mesh: the segment ends take the places of dr2 and h
+        // (part_int.cpp:42-43), the segment centre that of the position and
+        // the kernel spans half the segment (absorption.cpp:221-237,
+        // singleabs.h:84-88)
+        const float2 sg = A.seg[kk];
+        const double lo = (double)sg.x;
+        if (lo > 2 * A.vbox / A.velfac || (double)sg.y > 2 * A.vbox / A.velfac) return false;
+        pos1 = (lo + (double)sg.y) / 2.;
+        vdr2 = A.velfac * lo;
+        vsmooth = A.velfac * (double)sg.y;
+        vhigh = (vdr2 > 0 && vsmooth > 0) ? (vsmooth - vdr2) / 2. : 0;
+    } else {
+        const double dr2 = pair_dr2(los, pp, A.box);
+        const float smooth = r0.w;
+        const float sm2 = smooth * smooth;
+        if ((double)sm2 - dr2 <= 0) return false;            // absorption.cpp:232
+        double val1 = A.velfac * dr2;
+        val1 *= A.velfac;
+        vdr2 = val1;
+        vsmooth = A.velfac * (double)smooth;
+        vhigh = vsmooth * vsmooth > vdr2 ? sqrt(vsmooth * vsmooth - vdr2) : 0;   // singleabs.h:83
+    }
+    const double vel = A.velfac * pos1 + (double)pvel;
     const double inv_b = 1.0 / b;
     // a = voigt_fac/b, amp = sigma_a/sqrt(pi) (c/1e5)/b (absorption.cpp:238,244),
     // by one reciprocal: within 2 ulp of the reference's quotients
     const double aa = L.voigt_fac * inv_b;
-    const double vhigh = vsmooth * vsmooth > vdr2 ? sqrt(vsmooth * vsmooth - vdr2) : 0;   // singleabs.h:83
     const double bintov = A.vbox / A.nbins;
     const double amp = L.sigma_a * (kLight / 1e5 / kSqrtPi) * inv_b;
     const double dvk = 2. * vhigh / kNGrid;
@@ -495,15 +549,16 @@ __device__ inline float sample_weight(const DepositArgs &A, const PairConsts &C,
 // Set up one pair in the wave's LDS slot; returns false if the particle does
 // not reach the sightline.  Every lane of the wave calls it.
 template <int kG>
-__device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc &L, const LosInfo &los, int p,
+__device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc &L, const LosInfo &los, long long kk,
                                   PairTau &P, int lane, int max_sums = 1 << 30)
 {
     PairConsts C;
-    if (!pair_consts(A, line, L, los, p, max_sums, C)) return false;
+    if (!pair_consts(A, line, L, los, kk, max_sums, C)) return false;
+    const int p = A.pair_part[kk];
     const double bintov = A.vbox / A.nbins;
     wave_sync();   // the previous pair's constants are no longer read
     if (lane < kNGrid - 1) {
-        P.smp[lane].cv = ((lane + 1) * C.dvk - C.vhigh) * C.inv_b;
+        set_sample_offset(P.smp[lane], ((lane + 1) * C.dvk - C.vhigh) * C.inv_b);
         P.smp[lane].w = sample_weight(A, C, lane);
     }
     if (lane == 0) {
@@ -537,45 +592,104 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
     return true;
 }
 
-// The main kernel sets pairs up kQ at a time, one per lane, into a per-group
-// queue of compact records, so that the FP64 set-up arithmetic runs on kQ
-// lanes at once; activating a queued pair only expands its record into the
-// group's PairTau slot.  In 8-wavefront blocks kQ is 12 for 16-lane groups
-// (1% faster at c1 than 8; two blocks still fit a CU's LDS) and 16 for wider
-// groups, which are fewer per block (0.7% faster at c2); smaller blocks keep
-// 8, where more would cost a resident block per CU.
-template <int kG, int kBlock>
-constexpr int queue_depth() { return kBlock >= 512 ? (kG >= 32 ? kPairQueue : 12) : 8; }
+// The main kernel sets pairs up in batches, one pair per lane of a
+// wavefront, into a per-wavefront ring of compact records: the FP64 set-up
+// (pair constants, kernel weights and the ten far-wing coefficients) runs on
+// up to kWaveQueue lanes at once instead of on the few lanes of one lane
+// group, and a group taking the next record only expands it into its PairTau
+// slot.  The ring is shared by the wavefront's groups.
+constexpr int kWaveQueue = kWaveClaim;
 
 struct PairLite {
     double vel, inv_b, scale, vhigh, aa, inv_nm1;
     long long k;      // pair index
     float W[4];       // kernel weights of the centre sample and the 3 above it
+    float Cw[kWingTerms];   // far-wing coefficients (wing_coeffs)
     float ufar;
     int zmax;
     int shape;        // nm1 | rounds << 8 | pstep << 16 (step_shape)
     int pad;
 };
-static_assert(sizeof(PairLite) == 88, "PairLite layout");
+static_assert(sizeof(PairLite) == 128, "PairLite layout");
 
-// Claim the next A.claim (capped at kQ) pairs of item w for the group, set up one per lane, and
-// queue those walked in FP32; FP64 pairs go straight to the fix-up list.
-// Returns the number queued; `more` says whether the item has pairs left.
-template <int kG, int kQ, bool kStats>
-__device__ inline int refill_pairs(const DepositArgs &A, const WorkItem &w, int gitem, const LineDesc &L,
-                                   const LosInfo &los, unsigned long long *next, PairLite *q, int lane, bool &more,
-                                   LaneStats &st)
+// The far-wing coefficients of wing_setup for one pair, all ten on one lane
+// (the batched set-up of the main kernel), in FP32: every term is positive,
+// so the coefficients keep ~4e-7 of the FP64 values and the far-wing sum
+// ~4e-7 (checked against the FP64 computation over sub-sample counts 1..255,
+// kernel extents 0..30 b and a <= 0.04).  c0 = vhigh/b and h = half the
+// pixel width over b, both in units of ufar; W the kernel weights at offsets
+// 0, 1/4, 1/2, 3/4 of vhigh; m1 = sub-samples - 1 (at most FSAMD_WING_TMAX).
+__device__ __noinline__ void wing_coeffs(float c0, float h, int m1, float W0, float W1, float W2, float W3, float a,
+                                         float ufar, float *Cw)
 {
-    const int glane = lane % kG;
-    const int gleader = (lane / kG) * kG;
-    const int claim = A.claim < 1 ? 1 : (A.claim > kQ ? kQ : A.claim);
+    float Y[kWingTerms];
+    if (m1 == 0) {
+        Y[0] = 1.0f;
+#pragma unroll
+        for (int n = 1; n < kWingTerms; ++n) Y[n] = 0.0f;
+    } else {
+        const float h2 = h * h;
+        float pw = 1.0f;
+#pragma unroll
+        for (int n = 0; n < kWingTerms; ++n) {
+            Y[n] = pw * (float)c_wing_T[m1 * kWingTerms + n];
+            pw *= h2;
+        }
+    }
+    // kernel samples at +-(c0/4, c0/2, 3c0/4) and 0
+    const float x1 = 0.0625f * c0 * c0, x2 = 0.25f * c0 * c0, x3 = 0.5625f * c0 * c0;
+    const float w1 = 2.0f * W1, w2 = 2.0f * W2, w3 = 2.0f * W3;
+    float X[kWingTerms];
+    float p1 = 1.0f, p2 = 1.0f, p3 = 1.0f;
+#pragma unroll
+    for (int k = 0; k < kWingTerms; ++k) {
+        X[k] = w1 * p1 + w2 * p2 + w3 * p3 + (k == 0 ? W0 : 0.0f);
+        p1 *= x1;
+        p2 *= x2;
+        p3 *= x3;
+    }
+    float m[kWingTerms];
+#pragma unroll
+    for (int n = 0; n < kWingTerms; ++n) {
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k <= n; ++k) t += (float)c_binom_even[n * kWingTerms + k] * X[k] * Y[n - k];
+        m[n] = t;
+    }
+    const float iu = 1.0f / ufar, iu2 = iu * iu, u2 = ufar * ufar, a2 = a * a;
+    float lead = a * kInvSqrtPiF * iu2;   // a/sqrt(pi) ufar^(-2n-2) at n = 0
+#pragma unroll
+    for (int n = 0; n < kWingTerms; ++n) {
+        float cn = 0.0f, gs = lead;
+#pragma unroll
+        for (int j = 0; j <= n; ++j) {
+            if (j > 0) gs *= u2;
+            const int k = n - j;
+            const float qk = k > 0 ? (float)c_wing_q[k - 1] : 0.0f;
+            cn += gs * ((float)c_wing_p[k] - a2 * qk) * (float)c_binom_odd[n * kWingTerms + j] * m[j];
+        }
+        Cw[n] = cn;
+        lead *= iu2;
+    }
+}
+
+// Claim up to `want` pairs of item w for the wavefront, set up one per lane,
+// and append those walked in FP32 to the wavefront's ring (tail advances);
+// FP64 pairs go straight to the fix-up list.  Every lane of the wavefront
+// calls it; `more` says whether the item has pairs left.
+template <int kG, bool kStats>
+__device__ inline void refill_wave(const DepositArgs &A, const WorkItem &w, int gitem, const LineDesc &L,
+                                   const LosInfo &los, unsigned long long *next, PairLite *q, int lane, int want,
+                                   int &tail, bool &more, LaneStats &st)
+{
     unsigned long long base = 0;
-    if (glane == 0) base = atomicAdd(next, (unsigned long long)claim);
-    base = (unsigned long long)__shfl((long long)base, gleader, 64);
-    const long long kk = (long long)base + glane;
+    if (lane == 0) base = atomicAdd(next, (unsigned long long)want);
+    base = (unsigned long long)__shfl((long long)base, 0, 64);
+    const long long kk = (long long)base + lane;
+    const double bintov = A.vbox / A.nbins;
     bool enq = false;
     PairConsts C;
-    if (glane < claim && kk < w.p1 && pair_consts(A, w.line, L, los, A.pair_part[kk], kG * kRoundsMax, C)) {
+    if (lane < want && kk < w.p1 && pair_consts(A, w.line, L, los, kk, kG * kRoundsMax, C)) {
         if (kStats) st.pairs++;
         if (C.fp64) {
             if (kStats) st.pairs64++;
@@ -584,10 +698,10 @@ __device__ inline int refill_pairs(const DepositArgs &A, const WorkItem &w, int 
             enq = true;
         }
     }
-    const unsigned long long m = group_bits<kG>(__ballot(enq), lane);
-    wave_sync();   // the previous batch has been read
+    const unsigned long long m = __ballot(enq);
+    wave_sync();   // the records taken earlier have been read
     if (enq) {
-        PairLite r;
+        PairLite &r = q[(tail + __popcll(m & ((1ull << lane) - 1ull))) % kWaveQueue];
         r.vel = C.vel;
         r.inv_b = C.inv_b;
         r.scale = C.scale;
@@ -596,33 +710,41 @@ __device__ inline int refill_pairs(const DepositArgs &A, const WorkItem &w, int 
         r.inv_nm1 = C.np > 1 ? 1.0 / (C.np - 1) : 1.0;
         r.k = kk;
         const double inv_vs = 1.0 / C.vsmooth;
+        float W[4];
+#pragma unroll
         for (int i = 0; i < 4; ++i) {
             const double vv = (i + 4) * C.dvk - C.vhigh;
-            r.W[i] = (float)kernel_weight(A.kernel, sqrt(C.vdr2 + vv * vv) * inv_vs);
+            W[i] = (float)kernel_weight(A.kernel, sqrt(C.vdr2 + vv * vv) * inv_vs);
+            r.W[i] = W[i];
         }
-        r.ufar = (float)C.ufar;
+        const float ufar = (float)C.ufar;
+        r.ufar = ufar;
         r.zmax = C.zmax;
         const StepShape sh = step_shape(C.np, kG);
         r.shape = sh.nm1 | (sh.rounds << 8) | (sh.pstep << 16);
         r.pad = 0;
-        q[__popcll(m & ((1ull << glane) - 1ull))] = r;
+        // far-wing coefficients, in units of the ufar the walk compares with
+        const double iu = 1.0 / (double)ufar;
+        wing_coeffs((float)(C.vhigh * C.inv_b * iu), (float)(0.5 * bintov * C.inv_b * iu), sh.nm1, W[0], W[1], W[2],
+                    W[3], (float)C.aa, ufar, r.Cw);
     }
     wave_sync();
-    more = (long long)base + claim < w.p1;
-    return __popcll(m);
+    tail += __popcll(m);
+    more = (long long)base + want < w.p1;
 }
 
 // Expand queued record r into the group's PairTau slot (the fields the main
-// kernel's walk reads; setup_pair fills the same values).
+// kernel's walk reads; setup_pair and wing_setup fill the same values).
+// Only the lanes of the groups taking a record call it.
 template <bool kStats>
 __device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane, double bintov)
 {
-    wave_sync();   // the previous pair's constants are no longer read
     const double dvk = 2. * r.vhigh / kNGrid;
     if (glane < kNGrid - 1) {
-        P.smp[glane].cv = ((glane + 1) * dvk - r.vhigh) * r.inv_b;
+        set_sample_offset(P.smp[glane], ((glane + 1) * dvk - r.vhigh) * r.inv_b);
         P.smp[glane].w = r.W[glane < 3 ? 3 - glane : glane - 3];   // K(q) is even in the sample offset
     }
+    if (glane < kWingTerms) P.Cw[glane] = r.Cw[glane];
     if (glane == 0) {
         const int nm1 = r.shape & 0xff;
         P.vel = r.vel;
@@ -638,14 +760,13 @@ __device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane, d
         P.np = nm1 + 1;
         P.ufar = (double)r.ufar;
         P.ufar2 = (float)((double)r.ufar * (double)r.ufar);
-        P.wing_ready = 0;
+        P.wing_ready = 1;
         P.inv_nm1 = r.inv_nm1;
         P.dv = nm1 > 0 ? bintov * r.inv_nm1 : 0.0;
         P.nm1 = nm1;
         P.rounds = (r.shape >> 8) & 0xff;
         P.pstep = r.shape >> 16;
     }
-    wave_sync();
 }
 
 
@@ -930,12 +1051,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                        int use_queue)
 {
     constexpr int kGroups = kBlock / kG;
-    constexpr int kQ = queue_depth<kG, kBlock>();
+    constexpr int kWavesPerBlock = kBlock / 64;
+    constexpr int kGroupsPerWave = 64 / kG;
     extern __shared__ double smem[];
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
     __shared__ PairTau pslot[kGroups];
     __shared__ float sub_sums[kBlock * kRoundsMax];
-    __shared__ PairLite pqueue[kGroups * kQ];
+    __shared__ PairLite wqueue[kWavesPerBlock * kWaveQueue];
     __shared__ int s_item;
     __shared__ unsigned long long s_next;
     for (int i = threadIdx.x; i < kGTableSize; i += kBlock) gtab[i] = c_gtab[i];
@@ -944,7 +1066,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const int gid = threadIdx.x / kG;
     PairTau &P = pslot[gid];
     float *sbuf = sub_sums + gid * kG * kRoundsMax;
-    PairLite *q = pqueue + gid * kQ;
+    PairLite *q = wqueue + (threadIdx.x / 64) * kWaveQueue;
+    // the wavefront's leader lanes of its groups, below this lane's group
+    const unsigned long long leaders = kG == 64 ? 1ull : (kG == 32 ? 0x0000000100000001ull :
+                                                         0x0001000100010001ull);
+    const unsigned long long leaders_below = leaders & ((1ull << ((lane / kG) * kG)) - 1ull);
     const double bintov = A.vbox / A.nbins;
     const int half = A.nbins / 2;
     LaneStats st;
@@ -974,45 +1100,60 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         // group state: dir 0 = needs a pair, +1/-1 = walking, 2 = no pairs left
         long long k = 0;
         int dir = 0, c = 0, z0 = 0;
-        int qi = 0, qn = 0;      // the group's pair queue
+        // the wavefront's ring of set-up pairs: records head .. tail - 1
+        int head = 0, tail = 0;
         bool more = true;
         for (;;) {
             long long tc0 = 0;
+            const unsigned long long needing = __ballot(dir == 0) & leaders;
             if (kStats) {
                 tc0 = clock64();
-                const bool any = __ballot(dir == 0) != 0ull;
                 if (lane == 0) {
                     st.iters++;
-                    st.setup_iters += any;
+                    st.setup_iters += needing != 0ull;
                 }
             }
-            if (dir == 0) {
-                if (qi == qn && more) {
-                    qn = refill_pairs<kG, kQ, kStats>(A, w, gitem, L, los, &s_next, q, lane, more, st);
-                    qi = 0;
+            if (needing) {
+                const int nneed = __popcll(needing);
+                if (tail - head < nneed && more) {
+                    // claim up to a ring's worth, fewer near the end of the item so
+                    // that the block's wavefronts finish together
+                    const long long rem = (long long)w.p1 - (long long)__atomic_load_n(&s_next, __ATOMIC_RELAXED);
+                    const long long share = rem / (2 * kWavesPerBlock);
+                    int want = kWaveQueue - (tail - head);
+                    if (share < want) want = (int)(share > 2 * kGroupsPerWave ? share : 2 * kGroupsPerWave);
+                    if (A.claim < want) want = A.claim;
+                    if (want < 1) want = 1;
+                    want = __builtin_amdgcn_readfirstlane(want);
+                    refill_wave<kG, kStats>(A, w, gitem, L, los, &s_next, q, lane, want, tail, more, st);
                 }
-                if (qi < qn) {
-                    activate_pair<kStats>(q[qi], P, glane, bintov);
-                    k = q[qi].k;
-                    ++qi;
-                    dir = 1;
-                    z0 = P.zmax;
-                    c = 0;
-                } else if (!more) {
-                    dir = 2;
+                const int avail = tail - head;
+                const int rank = __popcll(needing & leaders_below);
+                bool fresh = false;
+                wave_sync();   // the previous pairs' constants are no longer read
+                if (dir == 0) {
+                    if (rank < avail) {
+                        const PairLite &r = q[(head + rank) % kWaveQueue];
+                        activate_pair<kStats>(r, P, glane, bintov);
+                        k = r.k;
+                        dir = 1;
+                        c = 0;
+                        fresh = true;
+                    } else if (!more) {
+                        dir = 2;
+                    }
                 }
+                wave_sync();
+                if (fresh) z0 = P.zmax;
+                head += nneed < avail ? nneed : avail;
             }
             const bool walking = (dir == 1 || dir == -1);
             if (!__ballot(dir != 2)) break;
             long long tc1 = 0;
             if (kStats) tc1 = clock64();
             // the step's first pixel in the far wing: |U| only grows along a walk
+            // (the record brought the far-wing coefficients)
             const bool far = walking && fabs(((z0 + dir * c) * bintov - P.vel + 0.5 * bintov) * P.inv_b) >= P.ufar;
-            if (far && !P.wing_ready) {
-                wing_setup<kG>(P, bintov, glane);
-                if (glane == 0) P.wing_ready = 1;
-                wave_sync();
-            }
             long long tc2 = 0;
             if (kStats) tc2 = clock64();
             int walked = 0, adv = 0;
@@ -1088,7 +1229,7 @@ __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, cons
         const LineDesc L = line_of(A, w.line);
         double *out_row = L.out + (size_t)w.los * A.nbins;
         const LosInfo los = los_info(A, w.los);
-        if (!setup_pair<kFixG>(A, w.line, L, los, A.pair_part[f.k], P, glane)) continue;
+        if (!setup_pair<kFixG>(A, w.line, L, los, f.k, P, glane)) continue;
         if (f.dir == 0) {
             const int lf = walk_fix<kFixG, kWalkFull64, kStats>(A, P, out_row, gtab, P.zmax, 1, half, lane, st, f.k,
                                                                   f.item);
@@ -1132,12 +1273,13 @@ __global__ __launch_bounds__(kThreads) void tau_fixup_kernel(DepositArgs A, cons
 __device__ inline void adj_inner32(const PairTau &P, double vouter, const float *gtab, float &S, float &SU,
                                    float &SF)
 {
-    const double s = vouter * P.inv_b;
+    const Split s = split_s(vouter * P.inv_b);
     const VoigtA va = P.va;
 #pragma unroll 2
     for (int i = 0; i < kNGrid - 1; ++i) {
-        const VoigtGrad g = voigt32_grad(va, P.smp[i].cv - s, gtab);
-        const float w = P.smp[i].w;
+        const KernelSample k = P.smp[i];
+        const VoigtGrad g = voigt32_grad(va, sample_u(k, s), gtab);
+        const float w = k.w;
         S = __builtin_fmaf(w, g.H, S);
         SU = __builtin_fmaf(w, g.Hu, SU);
         SF = __builtin_fmaf(w, g.F, SF);
@@ -1263,7 +1405,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             if (!__ballot(have)) break;
             if (!have) continue;
             const long long k = (long long)kk;
-            if (!setup_pair<kG>(A, w.line, L, los, A.pair_part[k], P, glane)) continue;
+            if (!setup_pair<kG>(A, w.line, L, los, k, P, glane)) continue;
             const long long wi = ((long long)w.line * A.npairs + k) * 2;
             const int lf = A.win[wi], lb = A.win[wi + 1];
             const int n = lf + lb;
@@ -1400,12 +1542,14 @@ int adjoint_blocks_per_cu(int gsize, int nbins)
 // ---------------------------------------------------------------------------
 // Column density
 
-// absorption.cpp:53-74 (cubic), 76-97 (quintic) / 109-115 (tophat)
+// absorption.cpp:53-74 (cubic), 76-97 (quintic) / 109-115 (tophat, also the
+// reference's default for any other id, absorption.cpp:136-148)
 __device__ inline double kern_frac(int kernel, double zlow, double zhigh, double smooth, double dr2,
                                    double zrange)
 {
     zlow = fmax(zlow, -zrange);
     zhigh = fmin(zhigh, zrange);
+    if (kernel == kVoronoiMesh) return fmax(0., zhigh - zlow);     // absorption.cpp:129-134
     if (kernel != kCubicSpline && kernel != kSphQuintic) return 3. / 4. / kPi * fmax(0., zhigh - zlow);
     if (zlow > zhigh) return 0;
     double total = kernel_weight(kernel, sqrt(dr2 + zlow * zlow) / smooth) / 2.;
@@ -1450,13 +1594,25 @@ __global__ __launch_bounds__(kThreads) void colden_kernel(DepositArgs A, const W
         for (long long k = w.p0 + threadIdx.x; k < w.p1; k += kThreads) {
             const int p = A.pair_part[k];
             const float *pp = &A.pos[3 * (size_t)p];
-            const double dr2 = pair_dr2(los, pp, A.box);
-            const float smooth = A.h[p];
             const float dens = L.dens[p];
-            const double pos1 = pp[los.ax - 1];
-            const float sm2 = smooth * smooth;
-            if ((double)sm2 - dr2 <= 0) continue;              // absorption.cpp:182
-            const double zrange = sqrt((double)sm2 - dr2);
+            double pos1 = pp[los.ax - 1], zrange, dr2;
+            float smooth;
+            if (A.seg) {
+                // Voronoi mesh: the segment ends replace dr2 and h
+                // (absorption.cpp:170-187): centre and half length
+                const float2 sg = A.seg[k];
+                dr2 = (double)sg.x;
+                smooth = sg.y;
+                if (dr2 > 2 * A.vbox / A.velfac || (double)smooth > 2 * A.vbox / A.velfac) continue;
+                pos1 = (dr2 + (double)smooth) / 2.;
+                zrange = ((double)smooth - dr2) / 2.;
+            } else {
+                dr2 = pair_dr2(los, pp, A.box);
+                smooth = A.h[p];
+                const float sm2 = smooth * smooth;
+                if ((double)sm2 - dr2 <= 0) continue;              // absorption.cpp:182
+                zrange = sqrt((double)sm2 - dr2);
+            }
             const int zlow = floor((pos1 - zrange) / boxtokpc);
             const int zhigh = ceil((pos1 + zrange) / boxtokpc);
             for (int z = zlow; z <= zhigh; ++z) {
@@ -1498,15 +1654,16 @@ __global__ __launch_bounds__(kThreads) void voigt_peak_kernel(long long per_thre
     double u = fmod((double)tid * du, umax);
     const double step = fmod((double)nthreads * du, umax);
     for (long long i = 0; i < per_thread; ++i) {
-        acc = __builtin_fmaf(w, voigt32(va, u, gtab), acc);
+        acc = __builtin_fmaf(w, voigt32(va, (float)u, gtab), acc);
         u += step;
         if (u >= umax) u -= umax;
     }
     if (acc == -1.0f) sink[0] = acc;   // keeps the loop live; never true
 }
 
-// H(a, u_i) by the deposit kernel's FP32 profile, one lane per point (the
-// profile's accuracy check); the damping parameter is set up as for a pair.
+// H(a, u_i) by the deposit kernel's FP32 profile at the points u_i rounded
+// to FP32, one lane per point (the profile's accuracy check); the damping
+// parameter is set up as for a pair.
 __global__ __launch_bounds__(kThreads) void voigt_eval_kernel(const double *u, long long n, double a, float *out)
 {
     __shared__ __attribute__((aligned(16))) float gtab[kGTableSize];
@@ -1514,7 +1671,7 @@ __global__ __launch_bounds__(kThreads) void voigt_eval_kernel(const double *u, l
     __syncthreads();
     const VoigtA va = make_voigt_a(a);
     for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
-        out[i] = voigt32(va, u[i], gtab);
+        out[i] = voigt32(va, (float)u[i], gtab);
 }
 
 int launch_voigt_eval(const double *u, long long n, double a, float *out, hipStream_t s)

@@ -17,6 +17,7 @@ constexpr double kPi = 3.14159265358979323846;
 
 constexpr int kTopHat = 0;       // singleabs.h:9
 constexpr int kCubicSpline = 1;  // singleabs.h:10
+constexpr int kVoronoiMesh = 2;  // singleabs.h:11
 constexpr int kSphQuintic = 3;   // singleabs.h:12
 constexpr int kNGrid = 8;        // singleabs.h:8
 
@@ -67,7 +68,7 @@ struct LineDesc {
 };
 
 constexpr int kMaxLines = 8;   // lines per deposit launch
-constexpr int kPairQueue = 16; // most pairs a lane group of the tau kernel sets up at once
+constexpr int kWaveClaim = 32; // most pairs a wavefront of the tau kernel sets up at once
 
 // One particle's inputs to the optical-depth kernels in one 64-byte record
 // (a single cache line per pair instead of one per input array): position,
@@ -109,11 +110,11 @@ struct DepositArgs {
     const double *cofm; // [L,3]
     const long long *line_off;  // [L+1]
     const int *pair_part;       // [M]
+    const float2 *seg;          // [M] Voronoi segment ends (kernel 2), else nullptr
     const LineDesc *lines;      // device [nlines], lines of this launch
     int nlines;
     int nitems;
-    int claim;                  // pairs a lane group claims at once (tau kernel; 1..kPairQueue, capped
-                                // at the kernel's queue depth)
+    int claim;                  // most pairs a wavefront of the tau kernel claims at once (1..kWaveClaim)
     int *queue;                 // work-queue counter (zeroed before launch)
     int *win;                   // optional [nlines][npairs][2] pixels walked per direction
     long long npairs;

@@ -8,32 +8,46 @@
 
 namespace fsamd {
 
-// Owning device allocation.
+// A bump allocator over device memory the caller owns (the allocation-free
+// entry's scratch array): buffers carved from it are never freed one by one.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+};
+
+// Device allocation: owned (hipMalloc / hipFree), or borrowed from an Arena.
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf();
-    int alloc(size_t n);
+    // n bytes from hipMalloc, or from the arena when one is given
+    int alloc(size_t n, Arena *arena = nullptr);
     void release();
     void take(DevBuf &o)
     {
         release();
         ptr = o.ptr;
         bytes = o.bytes;
+        borrowed = o.borrowed;
         o.ptr = nullptr;
         o.bytes = 0;
+        o.borrowed = false;
     }
     template <typename T>
     T *as() const { return (T *)ptr; }
-    // Hand the allocation over to the caller (who frees it), leaving this empty.
+    // Hand an owned allocation over to the caller (who frees it), leaving
+    // this empty; borrowed memory is not handed over (nullptr).
     void *disown()
     {
-        void *p = ptr;
+        void *p = borrowed ? nullptr : ptr;
         ptr = nullptr;
         bytes = 0;
+        borrowed = false;
         return p;
     }
 };
@@ -41,10 +55,17 @@ struct DevBuf {
 int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long long npart,
                const int *axis, const double *cofm, int nlos, hipStream_t s);
 int plan_near_lines(const fsamd_plan *P, int *out, long long *n_out, hipStream_t s);
+// Voronoi mesh: the segment ends of every pair (IndexTable::assign_cells).
+int plan_assign_cells(fsamd_plan *P, const float *pos, hipStream_t s);
+int near_lines_direct(double box, const float *pos, const float *h, long long npart, const int *axis,
+                      const double *cofm, int nlos, int *out, long long *n_out, hipStream_t s);
 
 }  // namespace fsamd
 
 struct fsamd_plan {
+    // Where the plan's device buffers come from: hipMalloc (nullptr), or an
+    // arena (the allocation-free entry, whose plans never call hipMalloc).
+    fsamd::Arena *arena = nullptr;
     double box = 0;
     long long npart = 0;
     int nlos = 0;
@@ -58,6 +79,8 @@ struct fsamd_plan {
     fsamd::DevBuf line_off;               // [nlos+1]
     fsamd::DevBuf pair_part;              // [npairs]
     fsamd::DevBuf order;                  // sightlines by descending pair count
+    fsamd::DevBuf seg;                    // [npairs] float2 Voronoi segment ends (kernel 2)
+    bool seg_ready = false;
     std::vector<int> order_host;
     std::vector<long long> off_host;
     // Deposit launch state, reused across calls so that a call allocates,
@@ -105,6 +128,7 @@ struct fsamd_plan {
     {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
-        for (void *p : retired) (void)hipFree(p);
+        for (void *p : retired)
+            if (p) (void)hipFree(p);
     }
 };

@@ -20,11 +20,15 @@
 //  form by selects, and every lane runs the same instructions, so the
 //  wavefront never diverges on |u|.
 //
-// u arrives in FP64, and u^2 and the exponent of exp(a^2 - u^2) stay in FP64
-// because exp amplifies the relative error of u by 2u^2; only 2^f on
-// |f| <= 1/2 runs in FP32 (the hardware exp2).  Measured against
-// scipy.special.wofz (the FP32 arithmetic emulated on the CPU): <= 3.6e-7
-// relative for a <= 0.04, 0 <= u <= 3000.
+// u arrives in FP32.  The deposit forms it as (c_hi - s_hi) + (c_lo - s_lo)
+// from two-float splits of the kernel-sample offset c and of the sub-sample
+// velocity s (s from the pixel's FP64 position), so u carries ~1 ulp relative
+// error even where c and s nearly cancel; u^2 and the exponent of
+// exp(a^2 - u^2) are FP32, and exp amplifies their relative error by ~2u^2
+// ln2: the Gaussian term carries ~1.2e-7 u^2 relative error on top of the
+// expansions' 3.6e-7 (measured against scipy.special.wofz, a <= 0.04,
+// 0 <= u <= 3000).  Summed over a pixel's samples this stays two orders
+// below the 1e-5 tau tolerance and the 2e-5 tail-cut guard band.
 //
 // An FP64 Faddeeva w(z) (voigt64) backs the FP32 path: pixels whose tail-cut
 // decision sits within a guard band of tautail, and pairs with a > kA32Max.
@@ -45,6 +49,7 @@ constexpr float kInvSqrtPiF = 0.56418958354775628695f;
 constexpr double kSqrtPi = 1.77245385090551602729;
 constexpr double kTwoOverSqrtPi = 1.12837916709551257390;
 constexpr double kLog2E = 1.44269504088896340736;
+constexpr float kLog2EF = 1.44269504088896340736f;
 constexpr int kGTableSize = FSAMD_G_NROW * FSAMD_G_STRIDE;
 
 // (2n+1)!!/2^n, n = 0..8
@@ -61,7 +66,7 @@ struct VoigtA {
     float a_sqpi;       // a/sqrt(pi)
     float a3_sqpi3;     // a^3/(3 sqrt(pi))
     float four_a2;      // 4a^2
-    double a2_log2e;    // a^2 log2(e)
+    float a2_log2e;     // a^2 log2(e)
 };
 
 __device__ inline VoigtA make_voigt_a(double a)
@@ -74,31 +79,27 @@ __device__ inline VoigtA make_voigt_a(double a)
     v.a_sqpi = (float)(a * (1.0 / kSqrtPi));
     v.a3_sqpi3 = (float)(a * a * a * (1.0 / (3.0 * kSqrtPi)));
     v.four_a2 = (float)(4.0 * a * a);
-    v.a2_log2e = a * a * kLog2E;
+    v.a2_log2e = (float)(a * a * kLog2E);
     return v;
 }
 
-// H(a, u) with u given in FP64 (the sample's offset from the line centre in
-// units of b), any u.  The table `tab` is 16-byte aligned.
-__device__ inline float voigt32(const VoigtA &v, double u64, const float *tab)
+// H(a, u) with u the sample's offset from the line centre in units of b,
+// any u.  The table `tab` is 16-byte aligned.
+__device__ inline float voigt32(const VoigtA &v, float u, const float *tab)
 {
     static_assert(FSAMD_G_DEG == 6 && FSAMD_G_STRIDE == 8, "voigt32 evaluates 7 coefficients per row");
-    const float uh = (float)__builtin_fabs(u64);
-    // exp(a^2 - u^2) = 2^y, y = -(u^2 - a^2) log2(e) in FP64: 2^(y - rint(y))
-    // in FP32 and the integer part by ldexp (0 once y passes the FP32 range)
-    const double u2 = u64 * u64;
-    const double y = __builtin_fmax(__builtin_fma(u2, -kLog2E, v.a2_log2e), -512.0);   // -(u^2 - a^2) log2(e)
-    const double n = __builtin_rint(y);
-    const float E = __builtin_ldexpf(__builtin_amdgcn_exp2f((float)(y - n)), (int)n);
+    const float uh = __builtin_fabsf(u);
+    const float p = u * u;
+    // exp(a^2 - u^2) = 2^y, y = -(u^2 - a^2) log2(e); 0 once y passes the FP32 range
+    const float E = __builtin_amdgcn_exp2f(__builtin_fmaf(p, -kLog2EF, v.a2_log2e));
     // cos(2au) to (2au)^6: the next term, (2au)^8/40320, is below 2e-8 for
     // a <= 0.04 where exp(-u^2) counts (u <= 5); u^2 clamped to 144
-    const float c2 = v.four_a2 * __builtin_fminf((float)u2, 144.0f);
+    const float c2 = v.four_a2 * __builtin_fminf(p, 144.0f);
     float cs = __builtin_fmaf(c2, -1.0f / 720.0f, 1.0f / 24.0f);
     cs = __builtin_fmaf(c2, -cs, 0.5f);
     cs = __builtin_fmaf(c2, -cs, 1.0f);
     const float re_exp = E * cs;
     // G: table piece (|u| < 7) or x R(x) (wing row), selected per lane
-    const float p = (float)u2;
     const float x = __builtin_amdgcn_rcpf(p);
     const bool core = uh < kVoigtCoreMax;
     const float q = uh * (1.0f / FSAMD_G_WIDTH);
@@ -138,11 +139,10 @@ struct VoigtGrad {
     float H, Hu, F;
 };
 
-__device__ inline VoigtGrad voigt32_grad(const VoigtA &v, double u64, const float *gtab)
+__device__ inline VoigtGrad voigt32_grad(const VoigtA &v, float u, const float *gtab)
 {
-    const float sgn = u64 < 0 ? -1.0f : 1.0f;
-    const double au = u64 < 0 ? -u64 : u64;
-    const float uh = (float)au;
+    const float sgn = u < 0 ? -1.0f : 1.0f;
+    const float uh = __builtin_fabsf(u);
     VoigtGrad r;
     if (uh >= kVoigtCoreMax) {
         const float x = __builtin_amdgcn_rcpf(uh * uh);
@@ -162,12 +162,9 @@ __device__ inline VoigtGrad voigt32_grad(const VoigtA &v, double u64, const floa
         r.F = v.a_sqpi * x * f;
         return r;
     }
-    // exp(a^2 - u^2) as in voigt32: the exponent in FP64, 2^frac in FP32
-    const double u2 = au * au;
-    const double y = __builtin_fmax(__builtin_fma(u2, -kLog2E, v.a2_log2e), -512.0);
-    const double ny = __builtin_rint(y);
-    const float E = __builtin_ldexpf(__builtin_amdgcn_exp2f((float)(y - ny)), (int)ny);
-    const float p = (float)u2;
+    // exp(a^2 - u^2) as in voigt32
+    const float p = uh * uh;
+    const float E = __builtin_amdgcn_exp2f(__builtin_fmaf(p, -kLog2EF, v.a2_log2e));
     const float c = 2.0f * v.a * uh;
     const float c2 = c * c;
     float cs = __builtin_fmaf(c2, -1.0f / 40320.0f, 1.0f / 720.0f);

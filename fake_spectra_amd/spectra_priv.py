@@ -136,6 +136,12 @@ class Plan:
         p = torch.as_tensor(_DeviceArray(parts.value, (m,), "<i4"), device=self.pos.device)
         return [p[int(o[s]):int(o[s + 1])].cpu().numpy() for s in sightlines]
 
+    def assign_cells(self):
+        """Voronoi mesh (kernel 2): cut each sightline into the segments its
+        particles own (IndexTable::assign_cells, index_table.cpp:152-223);
+        needed before tau / colden with kernel 2."""
+        N.check(N.lib().fsamd_plan_assign_cells(self.handle, _ptr(self.pos), _stream()))
+
     def keep_windows(self, enable: bool = True):
         """Record the deposited window of every (line, pair, direction) in the
         next tau() call, as the adjoint needs."""
@@ -153,6 +159,22 @@ class Plan:
         n = C.c_longlong()
         N.check(N.lib().fsamd_plan_near_lines(self.handle, _ptr(out), C.byref(n), _stream()))
         return out[: n.value]
+
+
+def near_lines(box: float, pos: torch.Tensor, h: torch.Tensor, axis: torch.Tensor, cofm: torch.Tensor) -> torch.Tensor:
+    """Sorted ids (int32 device tensor) of particles whose support reaches at
+    least one sightline, without building the neighbour lists (the work of
+    Py_near_lines, py_module.cpp:25-99, on device tensors)."""
+    pos = _require(pos, torch.float32, "pos")
+    h = _require(h, torch.float32, "h")
+    axis = _require(axis, torch.int32, "axis")
+    cofm = _require(cofm, torch.float64, "cofm")
+    n = int(pos.shape[0]) if pos.dim() >= 1 else 0
+    out = torch.empty(max(n, 1), dtype=torch.int32, device=pos.device)
+    cnt = C.c_longlong()
+    N.check(N.lib().fsamd_near_lines(float(box), _ptr(pos), _ptr(h), n, _ptr(axis), _ptr(cofm), int(axis.numel()),
+                                     _ptr(out), C.byref(cnt), _stream()))
+    return out[: cnt.value]
 
 
 def _line_structs(lines, dens_list, outs):
@@ -376,8 +398,9 @@ def _Particle_Interpolate(*args):
         raise ValueError("pos must have shape (N, 3)")
     if compute_tau and (vel.size < 3 * npart or temp.size < npart):
         raise ValueError("vel and temp must have one entry per particle when compute_tau is set")
-    if kernel not in (0, 1, 3):
-        raise ValueError("kernel %d is not supported: 0 (tophat), 1 (cubic SPH) and 3 (quintic SPH) are" % kernel)
+    # 0 tophat, 1 cubic SPH, 2 Voronoi mesh, 3 quintic SPH; any other id takes
+    # the reference's fallbacks (unweighted kernel samples in tau, the tophat
+    # fraction in column density; singleabs.h:158-163, absorption.cpp:136-148)
     if numlos and (axis.min() < 1 or axis.max() > 3):
         raise ValueError("axis values must be 1, 2 or 3")
     if nbins < 1:
@@ -389,6 +412,8 @@ def _Particle_Interpolate(*args):
     th = _to_dev(h, dev)
     plan = Plan(box100, tpos, th, _to_dev(axis, dev), _to_dev(cofm, dev))
     try:
+        if kernel == 2:
+            plan.assign_cells()     # part_int.cpp:28 (the mesh's segments)
         td = _to_dev(dens, dev)
         if compute_tau:
             out = tau(plan, nbins, kernel, velfac, _to_dev(vel.reshape(-1)[: 3 * npart], dev),
@@ -428,12 +453,7 @@ def _near_lines(*args):
     if numlos == 0 or npart == 0:
         return np.zeros(0, dtype=np.int32)
     dev = device()
-    plan = Plan(box100, _to_dev(pos, dev), _to_dev(hh, dev), _to_dev(axis, dev), _to_dev(cofm, dev))
-    try:
-        res = plan.near_lines().cpu().numpy()
-    finally:
-        plan.close()
-    return res
+    return near_lines(box100, _to_dev(pos, dev), _to_dev(hh, dev), _to_dev(axis, dev), _to_dev(cofm, dev)).cpu().numpy()
 
 
 def _rescale_mean_flux(*args):

@@ -14,7 +14,8 @@
  * Spectra.get_tau / Spectra.get_col_density:
  *   fsamd_plan_create        IndexTable ctor + get_near_particles
  *                            (fake_spectra/index_table.cpp:7-18, 130-150)
- *   fsamd_plan_near_lines    _near_lines (fake_spectra/py_module.cpp:25-99)
+ *   fsamd_near_lines,        _near_lines (fake_spectra/py_module.cpp:25-99)
+ *   fsamd_plan_near_lines
  *   fsamd_tau                ParticleInterp::compute_tau (part_int.cpp:20-51)
  *                            for several lines at once
  *   fsamd_colden             ParticleInterp::compute_colden (part_int.cpp:53-84)
@@ -60,6 +61,24 @@ int fsamd_plan_pairs(const fsamd_plan *plan, const long long **offsets, const in
  * to out (device, capacity npart); *n_out receives the count (host).
  * Replaces Py_near_lines (py_module.cpp:25-99). */
 int fsamd_plan_near_lines(const fsamd_plan *plan, int *out, long long *n_out, void *stream);
+/* The same list without building neighbour lists (what Py_near_lines,
+ * py_module.cpp:25-99, computes): the sightline grid, then one acceptance
+ * test per particle that stops at the first sightline it reaches, and a
+ * compaction.  pos, h, axis, cofm: device, as in fsamd_plan_create; out:
+ * device, capacity npart; *n_out: host.  Synchronizes the stream. */
+int fsamd_near_lines(double box, const float *pos, const float *h, long long npart, const int *axis,
+                     const double *cofm, int nlos, int *out, long long *n_out, void *stream);
+
+/* Voronoi mesh (kernel 2): cut every sightline of the plan into segments,
+ * one per particle of its neighbour list, at the points 0.1 apart along the
+ * line that each particle is the nearest to (IndexTable::assign_cells,
+ * index_table.cpp:152-223, with its periodic wrap).  Needed before
+ * fsamd_tau / fsamd_colden / fsamd_tau_adjoint with kernel 2, whose pairs then
+ * take the segment ends in place of dr2 and h (part_int.cpp:28-46).  pos:
+ * device [npart,3], the plan's particles.  Returns FSAMD_ERR_ARG where the
+ * reference exits (a particle owning two separate runs of one sightline).
+ * Synchronizes the stream. */
+int fsamd_plan_assign_cells(fsamd_plan *plan, const float *pos, void *stream);
 
 /* One absorption line of a multi-line call: the (lambda, gamma, fosc,
  * amumass) quadruple of _Particle_Interpolate with its species density. */
@@ -74,8 +93,9 @@ typedef struct fsamd_line {
 
 /* Optical depth of nlines lines (ParticleInterp::compute_tau,
  * part_int.cpp:20-51, with LineAbsorption::add_tau_particle,
- * absorption.cpp:212-279).  kernel: 0 tophat, 1 cubic spline, 3 quintic spline
- * (singleabs.h:9-12; the Voronoi mesh, 2, is not supported).
+ * absorption.cpp:212-279).  kernel: 0 tophat, 1 cubic spline, 2 Voronoi mesh
+ * (after fsamd_plan_assign_cells), 3 quintic spline (singleabs.h:9-12); any
+ * other id leaves the kernel samples unweighted, as the reference does.
  * mode: FSAMD_MODE_BATCHED or FSAMD_MODE_PLAIN. */
 int fsamd_tau(fsamd_plan *plan, int nbins, int kernel, double velfac, double tautail,
               const float *pos, const float *vel, const float *temp, const float *h,

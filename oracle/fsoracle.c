@@ -41,7 +41,9 @@
 #define NGRID 8                 /* singleabs.h:8 */
 #define TOP_HAT_KERNEL 0        /* singleabs.h:9 */
 #define SPH_CUBIC_SPLINE 1      /* singleabs.h:10 */
+#define VORONOI_MESH 2          /* singleabs.h:11 */
 #define SPH_QUINTIC_SPLINE 3    /* singleabs.h:12 */
+#define RESO 0.1                /* index_table.h:8, grid spacing of the mesh assignment */
 
 /* absorption.cpp:21-33, absorption.h:4 */
 #define SIGMA_T 6.652458558e-25
@@ -201,11 +203,14 @@ double fso_sph_quintic_kernel(double q)
     return norm * f;
 }
 
+/* singleabs.h:158-163: the weight of one kernel sample; an id that matches
+   none of the branches leaves the sample unweighted */
 double fso_kernel_weight(int kernel, double q)
 {
     if (kernel == SPH_CUBIC_SPLINE) return fso_sph_cubic_kernel(q);
     if (kernel == SPH_QUINTIC_SPLINE) return fso_sph_quintic_kernel(q);
-    return 3. / 4. / M_PI;
+    if (kernel == TOP_HAT_KERNEL) return 3. / 4. / M_PI;
+    return 1.0;
 }
 
 /* absorption.cpp:53-74 (cubic), 76-97 (quintic) and 109-115 (tophat):
@@ -214,6 +219,7 @@ double fso_kern_frac(int kernel, double zlow, double zhigh, double smooth, doubl
 {
     zlow = fmax(zlow, -zrange);
     zhigh = fmin(zhigh, zrange);
+    if (kernel == VORONOI_MESH) return fmax(0., zhigh - zlow);           /* absorption.cpp:129-134 */
     if (kernel != SPH_CUBIC_SPLINE && kernel != SPH_QUINTIC_SPLINE) return 3. / 4. / M_PI * fmax(0., zhigh - zlow);
     if (zlow > zhigh) return 0;
     double total = fso_kernel_weight(kernel, sqrt(dr2 + zlow * zlow) / smooth) / 2.;
@@ -243,6 +249,9 @@ static void absorber_init(absorber_t *s, double b, double vdr2, double vsm, doub
     s->aa = aa;
     s->kernel = kernel;
     s->vhigh = vsm * vsm > vdr2 ? sqrt(vsm * vsm - vdr2) : 0;   /* singleabs.h:83 */
+    /* singleabs.h:84-88: on the mesh, vdr2 and vsm are the segment's lower and
+       upper ends in velocity, and the kernel spans half the segment */
+    if (kernel == VORONOI_MESH) s->vhigh = (vdr2 > 0 && vsm > 0) ? (vsm - vdr2) / 2. : 0;
     s->samples = 0;
 }
 
@@ -308,8 +317,16 @@ static void add_colden(const line_t *l, double *colden, int nbins, double dr2, f
                        float pos, float smooth, counters_t *c)
 {
     double pos1 = pos;
-    if (smooth * smooth - dr2 <= 0) return;                      /* absorption.cpp:182 */
-    double zrange = sqrt(smooth * smooth - dr2);
+    double zrange;
+    if (l->kernel == VORONOI_MESH) {
+        /* absorption.cpp:170-186: dr2 and smooth are the segment's ends */
+        if (dr2 > 2 * l->vbox / l->velfac || smooth > 2 * l->vbox / l->velfac) return;
+        pos1 = (dr2 + smooth) / 2.;
+        zrange = (smooth - dr2) / 2.;
+    } else {
+        if (smooth * smooth - dr2 <= 0) return;                  /* absorption.cpp:182 */
+        zrange = sqrt(smooth * smooth - dr2);
+    }
     const double boxtokpc = l->vbox / nbins / l->velfac;
     const int zlow = floor((pos1 - zrange) / boxtokpc);
     const int zhigh = ceil((pos1 + zrange) / boxtokpc);
@@ -328,10 +345,16 @@ static void add_tau(const line_t *l, double *tau, int nbins, double dr2, float d
 {
     double pos1 = ppos;
     const double btherm = l->bfac * sqrt(temp);
-    if (smooth * smooth - dr2 <= 0) return;                      /* absorption.cpp:232 */
+    if (l->kernel == VORONOI_MESH) {
+        /* absorption.cpp:221-230: dr2 and smooth are the segment's ends */
+        if (dr2 > 2 * l->vbox / l->velfac || smooth > 2 * l->vbox / l->velfac) return;
+        pos1 = (dr2 + smooth) / 2.;
+    } else if (smooth * smooth - dr2 <= 0) {
+        return;                                                  /* absorption.cpp:232 */
+    }
     const double vel = l->velfac * pos1 + pvel;
     double val1 = l->velfac * dr2;
-    val1 *= l->velfac;
+    if (l->kernel != VORONOI_MESH) val1 *= l->velfac;           /* absorption.cpp:236-237 */
     absorber_t s;
     absorber_init(&s, btherm, val1, l->velfac * smooth, l->voigt_fac / btherm, l->kernel);
     const double bintov = l->vbox / nbins;
@@ -659,6 +682,60 @@ long long fso_near_lines(double box, const float *pos, const float *h, long long
 }
 
 /* ------------------------------------------------------------------------ */
+/* IndexTable::assign_cells (index_table.cpp:152-223): the Voronoi mesh's cut
+   of sightline `cofm3` (axis `axis`) into segments, one per particle of the
+   line's neighbour list parts[0..n).  Points at (i + 1/2) reso along the line
+   go to the nearest particle (periodic distance; the first in list order on
+   ties); a particle's segment runs from its first to its last point, +-reso/2.
+   The particle owning the first point may own the last points as well
+   (periodic wrap): at its first point past the middle of the box its segment
+   is restarted there and extended by a box length, and the walk stops.
+   seg[2k], seg[2k+1]: segment ends of particle k in FP32, 3 box - reso/2 and
+   3 box + reso/2 when it owns no point.  Returns 1 where the reference
+   exits: a particle owning two separate runs of points. */
+int fso_assign_cells(double box, const float *pos, const int *parts, long long n, int axis, const double *cofm3,
+                     float *seg)
+{
+    const int npts = (int)(box / RESO);
+    const double reso = box / npts;
+    for (long long k = 0; k < 2 * n; k++) seg[k] = 3 * box;
+    if (n == 0) return 0;
+    const double yp = cofm3[axis % 3], zp = cofm3[(axis + 1) % 3];
+    for (int i = 0; i < npts; i++) {
+        const double xp = (i + 0.5) * reso;
+        double min_dist = box;
+        long long min_ind = 0;
+        for (long long k = 0; k < n; k++) {
+            const float *pp = &pos[3 * (size_t)parts[k]];
+            double dx = fabs(pp[axis - 1] - xp);
+            if (dx > box / 2.) dx = box - dx;
+            double dy = fabs(pp[axis % 3] - yp);
+            if (dy > box / 2.) dy = box - dy;
+            double dz = fabs(pp[(axis + 1) % 3] - zp);
+            if (dz > box / 2.) dz = box - dz;
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist < min_dist) {
+                min_dist = dist;
+                min_ind = k;
+            }
+        }
+        if (seg[2 * min_ind] < reso && xp > box / 2. + 0.5 * reso) {
+            seg[2 * min_ind] = xp;
+            seg[2 * min_ind + 1] += box;
+            break;
+        }
+        if (xp > 1.5 * reso + seg[2 * min_ind + 1]) return 1;
+        if (seg[2 * min_ind] > 2 * box) seg[2 * min_ind] = xp;
+        seg[2 * min_ind + 1] = xp;
+    }
+    for (long long k = 0; k < n; k++) {
+        seg[2 * k] -= 0.5 * reso;
+        seg[2 * k + 1] += 0.5 * reso;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* ParticleInterp::compute_tau / compute_colden (part_int.cpp:20-84)         */
 
 static counters_t last_counters;
@@ -689,20 +766,34 @@ int fso_interpolate(int compute_tau, int nbins, int kernel, double box100, doubl
     line_t l;
     line_init(&l, lambda, gamma, fosc, amumass, velfac, box100, kernel, tautail);
     long long pairs = 0, tpix = 0, samp = 0, cpix = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : pairs, tpix, samp, cpix)
+    int failed = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : pairs, tpix, samp, cpix, failed)
     for (int i = 0; i < nlos; i++) {
         const int ax = axis[i];
         double *row = &out[(size_t)i * nbins];
         counters_t c = {0, 0, 0, 0};
+        const long long n = offs[i + 1] - offs[i];
+        float *seg = NULL;
+        if (kernel == VORONOI_MESH) {
+            seg = (float *)malloc(sizeof(float) * (size_t)(2 * n + 2));
+            if (fso_assign_cells(box100, pos, &parts[offs[i]], n, ax, &cofm[3 * (size_t)i], seg)) {
+                failed++;
+                free(seg);
+                continue;
+            }
+        }
         for (long long k = offs[i]; k < offs[i + 1]; k++) {
             const int p = parts[k];
             const float ppos = pos[3 * (size_t)p + ax - 1];
+            /* part_int.cpp:42-45: on the mesh, the segment ends replace dr2 and h */
+            const double d2 = seg ? (double)seg[2 * (k - offs[i])] : dr2[k];
+            const float sm = seg ? seg[2 * (k - offs[i]) + 1] : h[p];
             if (compute_tau)
-                add_tau(&l, row, nbins, dr2[k], dens[p], ppos, vel[3 * (size_t)p + ax - 1], temp[p],
-                        h[p], &c);
+                add_tau(&l, row, nbins, d2, dens[p], ppos, vel[3 * (size_t)p + ax - 1], temp[p], sm, &c);
             else
-                add_colden(&l, row, nbins, dr2[k], dens[p], ppos, h[p], &c);
+                add_colden(&l, row, nbins, d2, dens[p], ppos, sm, &c);
         }
+        free(seg);
         pairs += c.pairs;
         tpix += c.tau_pixels;
         samp += c.samples;
@@ -716,5 +807,5 @@ int fso_interpolate(int compute_tau, int nbins, int kernel, double box100, doubl
     free(parts);
     free(dr2);
     table_free(&tb);
-    return 0;
+    return failed ? 1 : 0;
 }

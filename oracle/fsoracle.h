@@ -40,6 +40,10 @@ int fso_interpolate(int compute_tau, int nbins, int kernel, double box100, doubl
                     double tautail, const float *pos, const float *vel, const float *dens,
                     const float *temp, const float *h, long long npart, const int *axis,
                     const double *cofm, int nlos, double *out);
+/* Voronoi mesh: segment ends (FP32, 2 per particle) of one sightline's
+ * neighbour list parts[0..n); returns 1 where the reference exits. */
+int fso_assign_cells(double box, const float *pos, const int *parts, long long n, int axis, const double *cofm3,
+                     float *seg);
 /* pairs, tau window pixels, Voigt samples, colden pixels of the last fso_interpolate */
 void fso_counters(long long *out4);
 

@@ -92,11 +92,7 @@ def test_shape_errors_are_value_errors():
     b[17] = np.ascontiguousarray(b[17][:, :2])
     with pytest.raises(ValueError):
         spectra_priv._Particle_Interpolate(*b)
-    b = list(a)
-    for kernel in (2, 4, -1):   # Voronoi mesh and unknown ids
-        b[2] = kernel
-        with pytest.raises(ValueError):
-            spectra_priv._Particle_Interpolate(*b)
+
 
 
 def test_empty_inputs_give_zero_arrays():

@@ -165,7 +165,8 @@ def test_adjoint_central_differences_full_workload(kernel):
     plan = SP.Plan(w.box, pos, hh, axis, cofm)
     F = _target(w, plan)
     xh = dev(w.neutral_fraction)
-    nfac = dev((w.dens["H1"] / w.neutral_fraction).astype(np.float32))
+    nfac_h = (w.dens["H1"] / w.neutral_fraction).astype(np.float32)
+    nfac = dev(nfac_h)
     dvel, dtemp = dev(w.vel), dev(w.temp)
     x = xh.clone().requires_grad_(True)
     T = dtemp.clone().requires_grad_(True)
@@ -216,8 +217,14 @@ def test_adjoint_central_differences_full_workload(kernel):
                 for rel in (1e-4, 5e-5):
                     h = rel * (max(abs(x0), 1.0) if name == "vel" else abs(x0))
                     xp, xm = np.float32(x0 + h), np.float32(x0 - h)
-                    quot.append((loss_of(name, float(xp)) - loss_of(name, float(xm))) /
-                                (float(xp) - float(xm)))
+                    dl = loss_of(name, float(xp)) - loss_of(name, float(xm))
+                    if name == "xhi":
+                        # the density x * nfac is rounded to FP32: divide by the
+                        # density step taken and apply d dens / dx = nfac
+                        nf = np.float32(nfac_h[p])
+                        quot.append(dl / (float(xp * nf) - float(xm * nf)) * float(nf))
+                    else:
+                        quot.append(dl / (float(xp) - float(xm)))
                 fd[name].append(quot)
             sub.close()
     finally:

@@ -117,6 +117,57 @@ def test_kernels_on_a_workload_match_oracle(oracle, kernel):
         assert_tau_close(o.cpu().numpy(), oracle.interpolate(*w.interp_args(ln, True)), "kernel %d %s" % (kernel, ln.name))
 
 
+@pytest.mark.parametrize("kernel", [5, -1])
+def test_unknown_kernel_ids_take_the_reference_fallbacks(oracle, kernel):
+    """An id other than 0-3 leaves the kernel samples unweighted in tau
+    (singleabs.h:158-163 take no branch) and takes the tophat fraction in
+    column density (absorption.cpp:136-148), as the reference does; the
+    oracle's fallback is pinned to the compiled reference on the CPU."""
+    w = synth.config(0, nlos=8)
+    for ct in (True, False):
+        args = list(w.interp_args(HI_1215 if ct else None, ct))
+        args[2] = kernel
+        got = SP._Particle_Interpolate(*args)
+        ref = oracle.interpolate(*args)
+        if ct:
+            assert_tau_close(got, ref, "kernel %d tau" % kernel)
+        else:
+            assert_colden_close(got, ref, "kernel %d colden" % kernel)
+
+
+def test_voronoi_mesh_matches_oracle(oracle):
+    """Kernel 2: the sightlines cut into the segments their particles own
+    (IndexTable::assign_cells) on the GPU, then the mesh branches of the tau
+    and column-density deposits, against the oracle (pinned to the compiled
+    reference); through the drop-in and through one plan serving two lines."""
+    w = synth.config(0, nlos=6)
+    for ct in (True, False):
+        args = list(w.interp_args(HI_1215 if ct else None, ct))
+        args[2] = 2
+        got = SP._Particle_Interpolate(*args)
+        ref = oracle.interpolate(*args)
+        assert np.any(ref > 0)
+        if ct:
+            assert_tau_close(got, ref, "Voronoi tau")
+        else:
+            assert_colden_close(got, ref, "Voronoi colden")
+    plan = plan_for(w)
+    plan.assign_cells()
+    d = dev(w.dens["H1"])
+    outs = SP.tau(plan, w.nbins, 2, w.velfac, dev(w.vel), dev(w.temp), [HI_1215, HI_1025], [d, d], w.tautail)
+    for ln, o in zip((HI_1215, HI_1025), outs):
+        args = list(w.interp_args(ln, True))
+        args[2] = 2
+        assert_tau_close(o.cpu().numpy(), oracle.interpolate(*args), "Voronoi plan %s" % ln.name)
+
+
+def test_voronoi_needs_assigned_cells():
+    w = synth.config(0, nlos=2, n_side=8)
+    plan = plan_for(w)
+    with pytest.raises(ValueError, match="assign_cells"):
+        SP.tau(plan, w.nbins, 2, w.velfac, dev(w.vel), dev(w.temp), [HI_1215], [dev(w.dens["H1"])])
+
+
 @pytest.mark.parametrize("g,waves", [(16, 4), (32, 4), (64, 4), (16, 2), (32, 8)])
 def test_group_sizes_match_oracle(oracle, g, waves):
     """Every lane-group width and wavefront count of the tau kernel gives
