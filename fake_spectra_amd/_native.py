@@ -28,7 +28,7 @@ EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan
            "fsamd_tau_autotune", "fsamd_plan_overflow", "fsamd_mean_flux_scale", "fsamd_res_corr",
            "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms", "fsamd_set_block_waves",
            "fsamd_voigt_eval", "fsamd_plan_window_generation", "fsamd_set_fp64",
-           "fsamd_near_lines", "fsamd_plan_assign_cells")
+           "fsamd_particle_interpolate_noalloc", "fsamd_near_lines", "fsamd_plan_assign_cells")
 
 
 class FsamdLine(C.Structure):
@@ -73,6 +73,8 @@ def _declare(L):
     L.fsamd_particle_interpolate.argtypes = [i, i, i, d, d, d, d, d, d, d, d, vp, vp, vp, vp, vp, ll, vp, vp,
                                              i, vp, vp]
     L.fsamd_particle_interpolate.restype = i
+    L.fsamd_particle_interpolate_noalloc.argtypes = L.fsamd_particle_interpolate.argtypes
+    L.fsamd_particle_interpolate_noalloc.restype = i
     L.fsamd_set_stats.argtypes = [i]
     L.fsamd_set_stats.restype = i
     L.fsamd_last_stats.argtypes = [C.POINTER(ll), i]

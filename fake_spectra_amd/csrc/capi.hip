@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace fsamd {
@@ -370,6 +371,38 @@ static int run_deposit(bool tau, fsamd_plan *P, int nbins, int kernel, double ve
     return FSAMD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Device scratch of the allocation-free entry (fsamd_particle_interpolate_noalloc):
+// a file-scope array in the code object, one copy per GPU, held until the
+// process ends.  Host code never names the array in device arguments: it
+// uses the address hipGetSymbolAddress returns for the current device, looked
+// up once per device on that device's first call.
+
+constexpr size_t kScratchBytes = FSAMD_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;                        // guards g_scratch_ptr
+static void *g_scratch_ptr[kMaxDevices];               // device address of `scratch`, per device
+static std::mutex g_scratch_call_mu[kMaxDevices];      // one call at a time per device
+
+static int scratch_for_device(int dev, void **ptr)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(FSAMD_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(FSAMD_ERR_HIP, std::string("hipGetSymbolAddress of the scratch array: ") + hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return FSAMD_OK;
+}
+
 // Arguments of a plan, checked before anything runs (fsamd_plan_create).
 static int check_plan_args(double box, const float *pos, const float *h, long long npart, const int *axis,
                            const double *cofm, int nlos)
@@ -490,6 +523,65 @@ int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double bo
         rc = fsamd_colden(plan, nbins, kernel, velfac, pos, h, &l, 1, stream);
     (void)hipStreamSynchronize((hipStream_t)stream);
     fsamd_plan_destroy(plan);
+    return rc;
+}
+
+int fsamd_particle_interpolate_noalloc(int compute_tau, int nbins, int kernel, double box100, double velfac,
+                                       double atime, double lambda, double gamma, double fosc, double amumass,
+                                       double tautail, const float *pos, const float *vel, const float *dens,
+                                       const float *temp, const float *h, long long npart, const int *axis,
+                                       const double *cofm, int nlos, double *out, void *stream)
+{
+    (void)atime;
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = check_plan_args(box100, pos, h, npart, axis, cofm, nlos)) return rc;
+    // per-particle and per-sightline buffers alone (packed records, pair
+    // offsets and counts; sightline tables, grid, row offsets and work items)
+    const double need = (double)npart * (sizeof(PartRec) + 2 * sizeof(long long) + sizeof(int)) +
+                        (double)nlos * (sizeof(int) * 4 + sizeof(double) * 3 + sizeof(double2) + sizeof(long long) +
+                                        sizeof(WorkItem));
+    if (need > (double)kScratchBytes)
+        return fail(FSAMD_ERR_UNSUPPORTED, "the allocation-free entry's scratch holds " +
+                                               std::to_string(kScratchBytes >> 20) + " MiB and " +
+                                               std::to_string(npart) + " particles and " + std::to_string(nlos) +
+                                               " sightlines need more; use fsamd_particle_interpolate");
+    int dev = 0;
+    FSAMD_HIP(hipGetDevice(&dev));
+    void *base = nullptr;
+    int rc = scratch_for_device(dev, &base);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> one_call(g_scratch_call_mu[dev]);
+    // no counters or events in this entry
+    const int stats_on = g_stats_on, timing_on = g_timing_on;
+    g_stats_on = 0;
+    g_timing_on = 0;
+    Arena arena;
+    arena.base = (char *)base;
+    arena.cap = kScratchBytes;
+    {
+        fsamd_plan P;
+        P.arena = &arena;
+        rc = plan_build(&P, box100, pos, h, npart, axis, cofm, nlos, s);
+        if (!rc && kernel == kVoronoiMesh) rc = plan_assign_cells(&P, pos, s);
+        if (!rc) {
+            fsamd_line l;
+            l.lambda_cm = lambda;
+            l.gamma = gamma;
+            l.fosc = fosc;
+            l.amumass = amumass;
+            l.dens = dens;
+            l.out = out;
+            rc = compute_tau ? run_deposit(true, &P, nbins, kernel, velfac, tautail, pos, vel, temp, h, &l, 1,
+                                           FSAMD_MODE_BATCHED, s)
+                             : run_deposit(false, &P, nbins, kernel, velfac, 0.0, pos, nullptr, nullptr, h, &l, 1,
+                                           FSAMD_MODE_BATCHED, s);
+        }
+        // the next call on this device reuses the scratch: drain this one first
+        const hipError_t e = hipStreamSynchronize(s);
+        if (!rc && e != hipSuccess) rc = fail(FSAMD_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    }
+    g_stats_on = stats_on;
+    g_timing_on = timing_on;
     return rc;
 }
 

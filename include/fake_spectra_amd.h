@@ -147,6 +147,30 @@ int fsamd_particle_interpolate(int compute_tau, int nbins, int kernel, double bo
                                const float *h, long long npart, const int *axis,
                                const double *cofm, int nlos, double *out, void *stream);
 
+/* fsamd_particle_interpolate without any device allocation.  No counterpart
+ * in the reference: it serves callers that cannot call hipMalloc (serving
+ * loops, harnesses that forbid allocation).  Same arguments, results and
+ * status codes as fsamd_particle_interpolate.  Every device buffer of the
+ * call (sightline grid, neighbour lists and their sort and scan space,
+ * packed particle records, work items, the fix-up list) is carved from one
+ * FSAMD_NOALLOC_SCRATCH_BYTES array in the library's device code, one copy
+ * per GPU, held until the process ends; a call that needs more returns
+ * FSAMD_ERR_UNSUPPORTED before touching it (about 100 bytes per particle,
+ * 16 per sightline-particle pair and 100 per sightline: e.g. 32^3 particles
+ * x 64 sightlines use ~4 MiB).  Calls on one GPU share the array: the
+ * library serialises them with a lock per device, and each call
+ * synchronises its stream before returning, so a caller must not launch
+ * work that still uses the array's contents; calls that overlapped on one
+ * GPU would race.  The counters and timing of fsamd_set_stats are not
+ * collected.  The output is undefined when a call fails. */
+#define FSAMD_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int fsamd_particle_interpolate_noalloc(int compute_tau, int nbins, int kernel, double box100,
+                                       double velfac, double atime, double lambda, double gamma,
+                                       double fosc, double amumass, double tautail, const float *pos,
+                                       const float *vel, const float *dens, const float *temp,
+                                       const float *h, long long npart, const int *axis,
+                                       const double *cofm, int nlos, double *out, void *stream);
+
 /* Counters of the last fsamd_tau call on this thread (host), FSAMD_NSTATS
  * entries: pairs deposited, pixels deposited, pixels re-evaluated in FP64 at
  * the tail cut, pairs evaluated entirely in FP64, Voigt samples of the
