@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-sightlines", type=int, default=1000,
                     help="sightlines of the CPU baseline's deposit sample (and of the parity check)")
     ap.add_argument("--graph-replays", type=int, default=2)
+    ap.add_argument("--tune", default="", metavar="GxW",
+                    help="lane-group width x wavefronts per sightline instead of autotuning (e.g. 32x8)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-near-lines", action="store_true")
     ap.add_argument("--no-drop-in", action="store_true")
@@ -350,10 +352,16 @@ def main():
     sel, sel_k = cpu_sample(w, args.cpu_sightlines)
     torch.cuda.synchronize()
 
-    # pick the lane-group width on this workload (kept by the plan for later calls)
-    tplan = SP.Plan(w.box, pos, hh, ax, cofm)
-    best_g, best_w, tune_ms = SP.autotune(tplan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail)
-    tplan.close()
+    # pick the lane-group width on this workload (kept by the plan for later calls),
+    # or take it from --tune GxW (counter runs: no sample launches of the same kernel)
+    if args.tune:
+        best_g, best_w = (int(x) for x in args.tune.lower().split("x"))
+        tune_ms = {}
+    else:
+        tplan = SP.Plan(w.box, pos, hh, ax, cofm)
+        best_g, best_w, tune_ms = SP.autotune(tplan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens,
+                                              w.tautail)
+        tplan.close()
     N.set_group_size(best_g)
     N.set_block_waves(best_w)
     progress("autotune: group width %d, %d wavefronts per sightline %s" % (best_g, best_w, tune_ms))
@@ -452,23 +460,24 @@ def main():
         "deposit_ms": dep_ms,
         "binning_ms": ms_per_step - dep_ms,
     }
-    # the same deposit captured once in a HIP graph and replayed
-    gplan = SP.Plan(w.box, pos, hh, ax, cofm)
-    graph = SP.TauGraph(gplan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    graph.replay()
-    torch.cuda.synchronize()
-    ev0.record()
-    for _ in range(args.graph_replays):
+    # the same deposit captured once in a HIP graph and replayed (--graph-replays 0 skips it)
+    if args.graph_replays > 0:
+        gplan = SP.Plan(w.box, pos, hh, ax, cofm)
+        graph = SP.TauGraph(gplan, w.nbins, w.kernel, w.velfac, vel, temp, w.lines, dens, w.tautail)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         graph.replay()
-    ev1.record()
-    torch.cuda.synchronize()
-    out["graph"] = {"deposit_ms": ev0.elapsed_time(ev1) / max(1, args.graph_replays),
-                    "replays": args.graph_replays, "overflow": graph.overflowed(),
-                    "note": "batched deposit + fix-up pass replayed from one captured hipGraph (binning excluded)"}
-    del graph
-    gplan.close()
-    progress("graph replays done")
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(args.graph_replays):
+            graph.replay()
+        ev1.record()
+        torch.cuda.synchronize()
+        out["graph"] = {"deposit_ms": ev0.elapsed_time(ev1) / args.graph_replays,
+                        "replays": args.graph_replays, "overflow": graph.overflowed(),
+                        "note": "batched deposit + fix-up pass replayed from one captured hipGraph (binning excluded)"}
+        del graph
+        gplan.close()
+        progress("graph replays done")
     out["autotune"] = {"group_width": best_g, "waves_per_sightline": best_w,
                        "sample_ms": {"%dx%d" % k: v for k, v in tune_ms.items()}}
     if not args.no_plain:

@@ -83,9 +83,10 @@ __global__ __launch_bounds__(kFluxThreads) void res_corr_kernel(const double *in
     }
 }
 
-// Per-sightline |DFT|^2 of d = exp(-s tau)/mean - 1 at k = 0..npix/2, by a
-// direct sum over an LDS twiddle table: power[row][k] = |sum_j d_j
-// exp(-2 pi i j k / npix)|^2.  One block per sightline.
+// Direct-sum power spectrum, kept for npix too large for the FFT's LDS
+// buffers: power[row][k] = |sum_j d_j exp(-2 pi i j k / npix)|^2 with
+// d = exp(-s tau)/mean - 1, over an LDS twiddle table.  One block per
+// sightline, O(npix^2).
 __global__ __launch_bounds__(kFluxThreads) void flux_dft_kernel(const double *tau, int nlos, int npix, double scale,
                                                                double inv_mean, double *power)
 {
@@ -117,6 +118,153 @@ __global__ __launch_bounds__(kFluxThreads) void flux_dft_kernel(const double *ta
             power[(size_t)row * nk + k] = re * re + im * im;
         }
     }
+}
+
+// ---- mixed-radix FFT power spectrum ---------------------------------------
+// Self-sorting (Stockham) decimation-in-time FFT of length n = R_1 R_2 ...
+// R_S in LDS, two buffers ping-ponged.  Stage s with radix R, after the
+// transforms of length p = R_1 ... R_{s-1}: for i in [0, n/R), k = i mod p,
+//   u_r = a[i + r n/R] W_n^{r k n/(p R)},   b[(i - k) R + k + q p] = sum_r u_r W_R^{r q}.
+// Two real sightlines a, b share one complex transform Z of a + i b; the sum
+// of their powers at mode k is (|Z_k|^2 + |Z_{n-k}|^2) / 2, which is all the
+// flux power needs (it averages the power over sightlines).
+
+constexpr int kFftMaxStages = 40;
+struct FftPlan {
+    int nst;
+    int radix[kFftMaxStages];
+};
+
+// Radices 4, 2, 3, 5, 7 first, then any remaining prime factors.
+static FftPlan fft_plan(int n)
+{
+    FftPlan f{};
+    for (int r : {4, 2, 3, 5, 7})
+        while (n % r == 0) { f.radix[f.nst++] = r; n /= r; }
+    for (int q = 11; n > 1; q += 2)
+        while (n % q == 0) { f.radix[f.nst++] = q; n /= q; }
+    return f;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b)
+{
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// One stage of radix R (compile-time), tw[m] = W_n^m = exp(-2 pi i m / n).
+template <int R>
+__device__ inline void fft_stage(const double2 *a, double2 *b, const double2 *tw, int n, int p)
+{
+    const int m = n / R;
+    const int tstep = n / (p * R);
+    for (int i = threadIdx.x; i < m; i += kFluxThreads) {
+        const int k = i % p;
+        double2 u[R];
+        u[0] = a[i];
+#pragma unroll
+        for (int r = 1; r < R; ++r) u[r] = cmul(a[i + r * m], tw[r * k * tstep]);  // r k tstep < n
+        const int j = (i - k) * R + k;
+        if constexpr (R == 2) {
+            b[j] = cadd(u[0], u[1]);
+            b[j + p] = csub(u[0], u[1]);
+        } else if constexpr (R == 4) {
+            // W_4 = -i
+            const double2 s02 = cadd(u[0], u[2]), d02 = csub(u[0], u[2]);
+            const double2 s13 = cadd(u[1], u[3]), d13 = csub(u[1], u[3]);
+            b[j] = cadd(s02, s13);
+            b[j + p] = make_double2(d02.x + d13.y, d02.y - d13.x);       // d02 - i d13
+            b[j + 2 * p] = csub(s02, s13);
+            b[j + 3 * p] = make_double2(d02.x - d13.y, d02.y + d13.x);   // d02 + i d13
+        } else {
+            const int wstep = n / R;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double2 acc = u[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(u[r], tw[((r * q) % R) * wstep]));
+                b[j + q * p] = acc;
+            }
+        }
+    }
+}
+
+// A stage of any radix R: one output per thread iteration, O(R) terms each.
+__device__ inline void fft_stage_any(const double2 *a, double2 *b, const double2 *tw, int n, int p, int R)
+{
+    const int m = n / R;
+    const int tstep = n / (p * R);
+    const int wstep = n / R;
+    for (int o = threadIdx.x; o < n; o += kFluxThreads) {
+        const int q = o / m, i = o - q * m;
+        const int k = i % p;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int r = 0; r < R; ++r) {
+            int t = r * k * tstep + ((r * q) % R) * wstep;   // both terms < n
+            if (t >= n) t -= n;
+            acc = cadd(acc, cmul(a[i + r * m], tw[t]));
+        }
+        b[(i - k) * R + k + q * p] = acc;
+    }
+}
+
+__global__ void fft_twiddle_kernel(double2 *tw, int n)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    double s, c;
+    sincospi(2.0 * m / n, &s, &c);
+    tw[m] = make_double2(c, -s);
+}
+
+// Each block transforms sightline pairs blockIdx.x, blockIdx.x + gridDim.x, ...
+// and writes the sum of their powers to part[blockIdx.x][k]; the blocks'
+// rows are then summed in block order, so the result does not depend on
+// scheduling.  LDS: two complex buffers of n and the accumulator of n/2 + 1.
+__global__ __launch_bounds__(kFluxThreads) void flux_fft_kernel(const double *tau, int nlos, int npix, double scale,
+                                                               double inv_mean, const double2 *tw, FftPlan fp,
+                                                               double *part)
+{
+    extern __shared__ double2 shc[];
+    double2 *A = shc, *B = shc + npix;
+    double *acc = reinterpret_cast<double *>(shc + 2 * npix);
+    const int nk = npix / 2 + 1;
+    for (int k = threadIdx.x; k < nk; k += kFluxThreads) acc[k] = 0.0;
+    const int npair = (nlos + 1) / 2;
+    for (int pr = blockIdx.x; pr < npair; pr += gridDim.x) {
+        const size_t r0 = 2 * (size_t)pr;
+        const bool two = r0 + 1 < (size_t)nlos;
+        __syncthreads();   // the last pair's power reads are done
+        for (int j = threadIdx.x; j < npix; j += kFluxThreads) {
+            const double x = exp(-scale * tau[r0 * npix + j]) * inv_mean - 1.0;
+            const double y = two ? exp(-scale * tau[(r0 + 1) * npix + j]) * inv_mean - 1.0 : 0.0;
+            A[j] = make_double2(x, y);
+        }
+        double2 *src = A, *dst = B;
+        int p = 1;
+        for (int st = 0; st < fp.nst; ++st) {
+            __syncthreads();
+            const int R = fp.radix[st];
+            switch (R) {
+            case 2: fft_stage<2>(src, dst, tw, npix, p); break;
+            case 3: fft_stage<3>(src, dst, tw, npix, p); break;
+            case 4: fft_stage<4>(src, dst, tw, npix, p); break;
+            case 5: fft_stage<5>(src, dst, tw, npix, p); break;
+            case 7: fft_stage<7>(src, dst, tw, npix, p); break;
+            default: fft_stage_any(src, dst, tw, npix, p, R); break;
+            }
+            double2 *t = src; src = dst; dst = t;
+            p *= R;
+        }
+        __syncthreads();
+        // each thread owns modes k = threadIdx.x + c kFluxThreads of acc
+        for (int k = threadIdx.x; k < nk; k += kFluxThreads) {
+            const double2 z = src[k], w = src[k == 0 ? 0 : npix - k];
+            acc[k] += 0.5 * ((z.x * z.x + z.y * z.y) + (w.x * w.x + w.y * w.y));
+        }
+    }
+    for (int k = threadIdx.x; k < nk; k += kFluxThreads) part[(size_t)blockIdx.x * nk + k] = acc[k];
 }
 
 // sum over rows of power[row][k], in row order (deterministic)
@@ -228,20 +376,45 @@ int fsamd_flux_power(const double *tau, int nlos, int npix, double scale, double
 {
     if (nlos < 1 || npix < 1 || !tau || !power || !(mean_flux != 0))
         return fail(FSAMD_ERR_ARG, "bad arguments to fsamd_flux_power");
-    const size_t shm = sizeof(double) * 3 * (size_t)npix;
-    if (shm > 160 * 1024) return fail(FSAMD_ERR_UNSUPPORTED, "flux power: npix too large for the LDS tables");
+    constexpr size_t kLdsBytes = 160 * 1024;
     hipStream_t s = (hipStream_t)stream;
     const int nk = npix / 2 + 1;
+    const size_t fft_shm = sizeof(double2) * 2 * (size_t)npix + sizeof(double) * (size_t)nk;
+    const size_t dft_shm = sizeof(double) * 3 * (size_t)npix;
+    const bool use_fft = fft_shm <= kLdsBytes;
+    if (!use_fft && dft_shm > kLdsBytes)
+        return fail(FSAMD_ERR_UNSUPPORTED, "flux power: npix too large for the LDS buffers");
+    // FFT: one partial row per block (at most 1024 blocks), DFT: one per sightline
+    const int npair = (nlos + 1) / 2;
+    const int nblk = use_fft ? std::min(npair, 1024) : nlos;
+    const size_t nrow = (size_t)nblk;
     double *per = nullptr;
-    FSAMD_HIP(hipMalloc(&per, sizeof(double) * (size_t)nlos * nk));
-    if (int rc = raise_lds_limit((const void *)flux_dft_kernel, shm)) {
-        (void)hipFree(per);
-        return rc;
+    FSAMD_HIP(hipMalloc(&per, sizeof(double) * nrow * nk + (use_fft ? sizeof(double2) * (size_t)npix : 0)));
+    hipError_t e = hipSuccess;
+    if (use_fft) {
+        double2 *tw = reinterpret_cast<double2 *>(per + nrow * nk);
+        if (int rc = raise_lds_limit((const void *)flux_fft_kernel, fft_shm)) {
+            (void)hipFree(per);
+            return rc;
+        }
+        fft_twiddle_kernel<<<(npix + 255) / 256, 256, 0, s>>>(tw, npix);
+        e = hipGetLastError();
+        if (e == hipSuccess) {
+            flux_fft_kernel<<<nblk, kFluxThreads, fft_shm, s>>>(tau, nlos, npix, scale, 1.0 / mean_flux, tw,
+                                                               fft_plan(npix), per);
+            e = hipGetLastError();
+        }
+    } else {
+        if (int rc = raise_lds_limit((const void *)flux_dft_kernel, dft_shm)) {
+            (void)hipFree(per);
+            return rc;
+        }
+        flux_dft_kernel<<<std::min(nlos, 65536), kFluxThreads, dft_shm, s>>>(tau, nlos, npix, scale,
+                                                                            1.0 / mean_flux, per);
+        e = hipGetLastError();
     }
-    flux_dft_kernel<<<std::min(nlos, 65536), kFluxThreads, shm, s>>>(tau, nlos, npix, scale, 1.0 / mean_flux, per);
-    hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        column_sum_kernel<<<(nk + 255) / 256, 256, 0, s>>>(per, nlos, nk, power);
+        column_sum_kernel<<<(nk + 255) / 256, 256, 0, s>>>(per, (int)nrow, nk, power);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);

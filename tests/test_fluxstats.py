@@ -68,3 +68,33 @@ def test_res_corr_known_answers():           # test_spectra.py testrescorr
     for i in (0, 1):
         assert abs(np.sum(tau2[i]) / np.sum(tau[i]) - 1) < 1e-6
         assert np.size(np.where(tau2[i] > 0)) == 15
+
+
+def test_observer_line_choice_each_branch():    # spectra.py:918-936
+    m = np.array([
+        # in band: the strongest of 0.5, 2.0 (5.0 is saturated)
+        [0.5, 5.0, 0.05, 0.05, 1.0],
+        [2.0, 7.0, 0.02, 0.05, 1.0],
+        [5.0, 4.0, 0.08, 0.01, 0.3],
+    ])
+    # col 1: nothing in band, the weakest above 0.1 -> 4.0 (line 2)
+    # col 2: nothing above 0.1, the strongest -> 0.08 (line 2)
+    # col 3: ties at the maximum -> the first (line 0)
+    # col 4: ties in band -> the first (line 0)
+    assert list(FS.observer_lines(m)) == [1, 2, 2, 0, 0]
+    # the band is open at both ends
+    assert list(FS.observer_lines(np.array([[0.1], [3.0], [0.09]]))) == [1]
+    assert list(FS.observer_lines(np.array([[0.1], [3.0], [2.99]]))) == [2]
+
+
+def test_device_line_choice_matches_oracle():
+    """products.observer_lines (torch, runs wherever the tensor lives) against
+    the one-sightline-at-a-time restatement, on values full of ties and band
+    edges."""
+    import torch
+    from fake_spectra_amd import products
+    rng = np.random.default_rng(3)
+    vals = np.array([0.02, 0.05, 0.1, 0.5, 2.0, 2.0, 3.0, 4.0, 9.0])
+    for _ in range(300):
+        m = rng.choice(vals, size=(int(rng.integers(1, 6)), 11))
+        np.testing.assert_array_equal(products.observer_lines(torch.from_numpy(m)).numpy(), FS.observer_lines(m))

@@ -2,7 +2,8 @@
 _Particle_Interpolate, py_module.cpp:103-233, on device pointers), bound
 through ctypes exactly as a non-Python caller binds it, on device buffers:
 tau and column density of the 32^3-particle, 64-sightline field against the
-oracle and the reference's own outputs, and its status codes."""
+oracle and the reference's own outputs, the Voronoi kernel, and its status
+codes."""
 import ctypes as C
 import os
 
@@ -62,11 +63,27 @@ def test_one_call_tau_and_colden_match_oracle_and_reference(oracle):
     assert_colden_close(col[z["sel"]], z["colden"], "c0 colden vs reference")
 
 
+def test_one_call_voronoi_kernel_matches_oracle(oracle):
+    """Kernel 2 through the one-call entry: it cuts the sightlines into the
+    particles' segments itself (IndexTable::assign_cells) before the deposit."""
+    lib = _lib()
+    w = synth.config(0, nlos=6)
+    for ct in (1, 0):
+        rc, got = _call(lib, ct, w, kernel=2)
+        assert rc == 0, lib.fsamd_last_error()
+        args = list(w.interp_args(HI_1215 if ct else None, bool(ct)))
+        args[2] = 2
+        ref = oracle.interpolate(*args)
+        assert np.any(ref > 0)
+        if ct:
+            assert_tau_close(got, ref, "one-call Voronoi tau")
+        else:
+            assert_colden_close(got, ref, "one-call Voronoi colden")
+
+
 def test_one_call_status_codes():
     lib = _lib()
     w = synth.config(0, nlos=4, n_side=8)
-    rc, _ = _call(lib, 1, w, kernel=2)
-    assert rc == N.FSAMD_ERR_UNSUPPORTED and b"kernel" in lib.fsamd_last_error()
     rc, _ = _call(lib, 1, w, nbins=0)
     assert rc == N.FSAMD_ERR_ARG and b"nbins" in lib.fsamd_last_error()
     rc, _ = _call(lib, 1, w, null_pos=True)

@@ -8,7 +8,7 @@ import torch
 
 import fluxstats as FS
 from fake_spectra_amd import fluxstatistics as GF, spectra_priv as SP, synth
-from gpu_util import dev, plan_for
+from gpu_util import TAU_REL, assert_tau_close, dev, plan_for
 
 pytestmark = pytest.mark.gpu
 
@@ -99,3 +99,49 @@ def test_res_corr_matches_scipy():
         assert isinstance(got, torch.Tensor)
         np.testing.assert_allclose(got.cpu().numpy(), FS.res_corr(t.cpu().numpy(), dv, fwhm), rtol=1e-12,
                                    atol=1e-14 * float(t.max()))
+
+
+def test_observer_tau_matches_oracle_choice(oracle):
+    """get_observer_tau (spectra.py:894-943) over five H I lines of different
+    strengths (Lyman alpha with its oscillator strength scaled by 0.05, 0.004
+    and 1e-4 besides the real 1215 and 1025 lines, so that different
+    sightlines choose different lines) on the 32^3-particle field: the line chosen per
+    sightline and its optical depth against the oracle's taus, smoothing and
+    choice.  Sightlines whose smoothed maxima sit within the tau tolerance of a band edge
+    or of each other could go either way and are left out of the index check."""
+    from fake_spectra_amd import products
+    from fake_spectra_amd.lines import HI_1215, HI_1025, Line
+    w = synth.config(0)
+    lines = [HI_1215, HI_1025] + [Line("scaled %g" % f, "H", 1, 1215.6701, 0.4164 * f, 6.265e8)
+                                  for f in (0.05, 0.004, 1e-4)]
+    dvbin = w.velfac * w.box / w.nbins
+    plan = plan_for(w)
+    got, choice = products.observer_tau(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), lines,
+                                        dev(w.dens_for(HI_1215)), dvbin, 8.0)
+    ref_taus = np.stack([oracle.interpolate(*w.interp_args(ln, True)) for ln in lines])
+    maxt = FS.res_corr(ref_taus, dvbin, 8.0).max(axis=-1)
+    ref_choice = FS.observer_lines(maxt)
+    choice = choice.cpu().numpy()
+    srt = np.sort(np.concatenate([maxt, np.full((1, maxt.shape[1]), 0.1), np.full((1, maxt.shape[1]), 3.0)]), axis=0)
+    close = (np.diff(srt, axis=0) <= 2 * TAU_REL * np.maximum(srt[1:], 1e-30)).any(axis=0)
+    assert close.mean() < 0.01
+    np.testing.assert_array_equal(choice[~close], ref_choice[~close])
+    assert len(np.unique(ref_choice)) >= 2, "the workload should exercise more than one line"
+    keep = np.nonzero(~close)[0]
+    assert_tau_close(got.cpu().numpy()[keep], ref_taus[ref_choice[keep], keep], "observer tau")
+    with pytest.raises(ValueError):
+        products.observer_tau(plan, w.nbins, w.kernel, w.velfac, dev(w.vel), dev(w.temp), lines,
+                              dev(w.dens_for(HI_1215)), dvbin, 0.0)
+
+
+@pytest.mark.parametrize("npix,nlos", [(1, 3), (2, 2), (97, 5), (210, 7), (572, 4), (1024, 33), (4096, 3), (6000, 2)])
+def test_flux_power_fft_lengths(npix, nlos):
+    """The mixed-radix FFT over lengths with radices 2, 3, 4, 5, 7, larger
+    primes (97, 11 and 13) and a lone sightline left over from the pairing;
+    6000 is past the FFT's LDS buffers and takes the direct sum."""
+    rng = np.random.default_rng(npix)
+    tau = rng.exponential(0.5, size=(nlos, npix))
+    kf, p = GF.flux_power(tau, 1000.0, mean_flux_desired=None)
+    rk, rp = FS.flux_power(tau, 1000.0)
+    np.testing.assert_allclose(kf, rk, rtol=1e-14)
+    np.testing.assert_allclose(p, rp, rtol=1e-10, atol=1e-13 * rp.max())
