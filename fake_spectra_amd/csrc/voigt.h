@@ -65,8 +65,9 @@ struct VoigtA {
     float two_a_sqpi;   // 2a/sqrt(pi)
     float a_sqpi;       // a/sqrt(pi)
     float a3_sqpi3;     // a^3/(3 sqrt(pi))
-    float four_a2;      // 4a^2
     float a2_log2e;     // a^2 log2(e)
+    // log2 of Re exp(-(u + ia)^2) = a2_log2e + lc1 u^2 + lc2 u^4 + lc3 u^6 (voigt32)
+    float lc1, lc2, lc3;
 };
 
 __device__ inline VoigtA make_voigt_a(double a)
@@ -78,8 +79,11 @@ __device__ inline VoigtA make_voigt_a(double a)
     v.two_a_sqpi = (float)(a * kTwoOverSqrtPi);
     v.a_sqpi = (float)(a * (1.0 / kSqrtPi));
     v.a3_sqpi3 = (float)(a * a * a * (1.0 / (3.0 * kSqrtPi)));
-    v.four_a2 = (float)(4.0 * a * a);
     v.a2_log2e = (float)(a * a * kLog2E);
+    const double a2 = a * a;
+    v.lc1 = (float)(-(1.0 + 2.0 * a2) * kLog2E);
+    v.lc2 = (float)(-(4.0 / 3.0) * a2 * a2 * kLog2E);
+    v.lc3 = (float)(-(64.0 / 45.0) * a2 * a2 * a2 * kLog2E);
     return v;
 }
 
@@ -90,15 +94,14 @@ __device__ inline float voigt32(const VoigtA &v, float u, const float *tab)
     static_assert(FSAMD_G_DEG == 6 && FSAMD_G_STRIDE == 8, "voigt32 evaluates 7 coefficients per row");
     const float uh = __builtin_fabsf(u);
     const float p = u * u;
-    // exp(a^2 - u^2) = 2^y, y = -(u^2 - a^2) log2(e); 0 once y passes the FP32 range
-    const float E = __builtin_amdgcn_exp2f(__builtin_fmaf(p, -kLog2EF, v.a2_log2e));
-    // cos(2au) to (2au)^6: the next term, (2au)^8/40320, is below 2e-8 for
-    // a <= 0.04 where exp(-u^2) counts (u <= 5); u^2 clamped to 144
-    const float c2 = v.four_a2 * __builtin_fminf(p, 144.0f);
-    float cs = __builtin_fmaf(c2, -1.0f / 720.0f, 1.0f / 24.0f);
-    cs = __builtin_fmaf(c2, -cs, 0.5f);
-    cs = __builtin_fmaf(c2, -cs, 1.0f);
-    const float re_exp = E * cs;
+    // Re exp(-(u + ia)^2) = exp(a^2 - u^2) cos(2au) = 2^y, with ln cos x =
+    // -x^2/2 - x^4/12 - x^6/45 - ...:  y = log2(e) (a^2 - (1 + 2a^2) u^2 -
+    // (4/3) a^4 u^4 - (64/45) a^6 u^6).  The dropped term, (17/2520)(2au)^8,
+    // changes H by < 5e-9 relative for a <= 0.04 (against scipy's wofz); the
+    // u^4 and u^6 terms are small next to the u^2 one, so y keeps the
+    // rounding of its first fma.  2^y is 0 once y passes the FP32 range.
+    const float y = __builtin_fmaf(p, v.lc1, v.a2_log2e);
+    const float re_exp = __builtin_amdgcn_exp2f(__builtin_fmaf(p * p, __builtin_fmaf(p, v.lc3, v.lc2), y));
     // G: table piece (|u| < 7) or x R(x) (wing row), selected per lane
     const float x = __builtin_amdgcn_rcpf(p);
     const bool core = uh < kVoigtCoreMax;
