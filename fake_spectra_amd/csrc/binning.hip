@@ -21,6 +21,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -808,6 +810,59 @@ int plan_assign_cells(fsamd_plan *P, const float *pos, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Host side
 
+// Device memory kept across plans.  Buffers recycled where no queued work can
+// still use them (after a synchronization: a plan's own buffers when it is
+// destroyed, the binning's temporaries at its end) go to a per-device free
+// list instead of hipFree, so that plans of one shape made one after another
+// (the drop-in's plan per call, a loop over snapshots) reuse the same blocks
+// instead of allocating and freeing some 20 GB of pair buffers each time.
+// Sizes are rounded (256 B, or 2 MiB above 2 MiB); a request takes the
+// smallest pooled block that fits with at most 1/4 + 2 MiB to spare.  The
+// pool keeps at most kPoolCap bytes per device; a hipMalloc that fails
+// empties the calling device's pool and tries again.
+namespace {
+constexpr size_t kPoolCap = 64ull << 30;
+constexpr size_t kBig = 2ull << 20;
+std::mutex g_pool_mu;
+std::map<int, std::multimap<size_t, void *>> g_pool;
+std::map<int, size_t> g_pool_bytes;
+
+size_t pool_round(size_t n) { return n >= kBig ? (n + kBig - 1) & ~(kBig - 1) : (n + 255) & ~(size_t)255; }
+
+bool pool_take(int dev, size_t n, void *&p, size_t &bytes)
+{
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    auto &m = g_pool[dev];
+    auto it = m.lower_bound(n);
+    if (it == m.end() || it->first > n + n / 4 + kBig) return false;
+    p = it->second;
+    bytes = it->first;
+    g_pool_bytes[dev] -= bytes;
+    m.erase(it);
+    return true;
+}
+
+bool pool_put(int dev, void *p, size_t bytes)
+{
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    if (g_pool_bytes[dev] + bytes > kPoolCap) return false;
+    g_pool[dev].emplace(bytes, p);
+    g_pool_bytes[dev] += bytes;
+    return true;
+}
+
+void pool_flush(int dev)
+{
+    std::multimap<size_t, void *> m;
+    {
+        std::lock_guard<std::mutex> g(g_pool_mu);
+        m.swap(g_pool[dev]);
+        g_pool_bytes[dev] = 0;
+    }
+    for (auto &e : m) (void)hipFree(e.second);
+}
+}  // namespace
+
 DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release()
@@ -816,6 +871,18 @@ void DevBuf::release()
     ptr = nullptr;
     bytes = 0;
     borrowed = false;
+    dev = -1;
+}
+
+void DevBuf::recycle()
+{
+    if (ptr && !borrowed && dev >= 0 && pool_put(dev, ptr, bytes)) {
+        ptr = nullptr;
+        bytes = 0;
+        dev = -1;
+        return;
+    }
+    release();
 }
 
 int DevBuf::alloc(size_t n, Arena *arena)
@@ -835,13 +902,26 @@ int DevBuf::alloc(size_t n, Arena *arena)
         arena->used = at + n;
         return FSAMD_OK;
     }
+    int d = 0;
+    FSAMD_HIP(hipGetDevice(&d));
+    n = pool_round(n);
+    if (pool_take(d, n, ptr, bytes)) {
+        dev = d;
+        return FSAMD_OK;
+    }
     hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        pool_flush(d);
+        e = hipMalloc(&ptr, n);
+    }
     if (e != hipSuccess) {
         ptr = nullptr;
         (void)hipGetLastError();
         return fail(FSAMD_ERR_OOM, "hipMalloc of " + std::to_string(n) + " bytes failed");
     }
     bytes = n;
+    dev = d;
     return FSAMD_OK;
 }
 
@@ -953,7 +1033,7 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     long long M = 0;
     FSAMD_HIP(hipMemcpyAsync(&M, P->poff.as<long long>() + npart, sizeof(long long), hipMemcpyDeviceToHost, s));
     FSAMD_HIP(hipStreamSynchronize(s));
-    cnt.release();
+    cnt.recycle();
     P->npairs = M;
     DevBuf k0, v0, k1, v1;
     const size_t mbytes = sizeof(int) * (size_t)std::max<long long>(M, 1);
@@ -970,9 +1050,9 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     const int passes = (total_bits + kRsMaxBits - 1) / kRsMaxBits;
     const int bits_per = (total_bits + passes - 1) / passes;
     DevBuf *kin = &k0, *vin = &v0, *kout = &k1, *vout = &v1;
+    DevBuf table, tws;
     if (M > 0) {
         const long long ntiles = (M + kRsTile - 1) / kRsTile;
-        DevBuf table, tws;
         if ((rc = table.alloc(sizeof(long long) * (size_t)((1 << bits_per) * ntiles + 1), P->arena))) return rc;
         if ((rc = tws.alloc(exclusive_scan_ws_bytes((1 << bits_per) * ntiles + 1), P->arena))) return rc;
         for (int ps = 0; ps < passes; ++ps) {
@@ -1004,13 +1084,23 @@ int plan_build(fsamd_plan *P, double box, const float *pos, const float *h, long
     FSAMD_HIP(hipMemcpyAsync(off.data(), P->line_off.ptr, sizeof(long long) * (nlos + 1),
                              hipMemcpyDeviceToHost, s));
     FSAMD_HIP(hipStreamSynchronize(s));
-    std::vector<int> order(nlos);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return off[a + 1] - off[a] > off[b + 1] - off[b];
-    });
+    // the binning's temporaries are idle now
+    for (DevBuf *b : {&scan_ws, kin, kout, vout, &table, &tws}) b->recycle();
     P->max_row = 0;
     for (int i = 0; i < nlos; ++i) P->max_row = std::max(P->max_row, off[i + 1] - off[i]);
+    std::vector<int> order(nlos);
+    if (P->max_row <= 4 * (long long)nlos + (1 << 20)) {
+        // counting sort on the pair count, descending, stable
+        std::vector<int> first((size_t)P->max_row + 2, 0);
+        for (int i = 0; i < nlos; ++i) ++first[(size_t)(P->max_row - (off[i + 1] - off[i])) + 1];
+        for (size_t c = 1; c < first.size(); ++c) first[c] += first[c - 1];
+        for (int i = 0; i < nlos; ++i) order[first[(size_t)(P->max_row - (off[i + 1] - off[i]))]++] = i;
+    } else {
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return off[a + 1] - off[a] > off[b + 1] - off[b];
+        });
+    }
     if ((rc = P->order.alloc(sizeof(int) * (size_t)std::max(nlos, 1), P->arena))) return rc;
     if (nlos > 0)
         FSAMD_HIP(hipMemcpy(P->order.ptr, order.data(), sizeof(int) * nlos, hipMemcpyHostToDevice));
@@ -1047,6 +1137,7 @@ int near_lines_direct(double box, const float *pos, const float *h, long long np
     long long cnt = 0;
     FSAMD_HIP(hipMemcpyAsync(&cnt, fscan.as<long long>() + npart, sizeof(long long), hipMemcpyDeviceToHost, s));
     FSAMD_HIP(hipStreamSynchronize(s));
+    for (DevBuf *b : {&flag, &fscan, &ws}) b->recycle();
     *n_out = cnt;
     return FSAMD_OK;
 }
@@ -1072,6 +1163,7 @@ int plan_near_lines(const fsamd_plan *P, int *out, long long *n_out, hipStream_t
     long long cnt = 0;
     FSAMD_HIP(hipMemcpyAsync(&cnt, fscan.as<long long>() + n, sizeof(long long), hipMemcpyDeviceToHost, s));
     FSAMD_HIP(hipStreamSynchronize(s));
+    for (DevBuf *b : {&flag, &fscan, &ws}) b->recycle();
     *n_out = cnt;
     return FSAMD_OK;
 }

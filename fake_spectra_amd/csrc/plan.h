@@ -3,6 +3,7 @@
 #include "fsamd_internal.h"
 #include "deposit.h"
 
+#include <deque>
 #include <memory>
 #include <vector>
 
@@ -16,27 +17,36 @@ struct Arena {
     size_t used = 0;
 };
 
-// Device allocation: owned (hipMalloc / hipFree), or borrowed from an Arena.
+// Device allocation: owned (from the device pool or hipMalloc, see
+// DevBuf::alloc), or borrowed from an Arena.
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
     bool borrowed = false;
+    int dev = -1;          // device of an owned allocation
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf();
-    // n bytes from hipMalloc, or from the arena when one is given
+    // at least n bytes: from the arena when one is given, else a block of
+    // the calling device's pool, else hipMalloc
     int alloc(size_t n, Arena *arena = nullptr);
+    // hipFree (which waits for the device)
     void release();
+    // back to the device's pool, for a later alloc; only where no queued work
+    // can still use the buffer (after a synchronization)
+    void recycle();
     void take(DevBuf &o)
     {
         release();
         ptr = o.ptr;
         bytes = o.bytes;
         borrowed = o.borrowed;
+        dev = o.dev;
         o.ptr = nullptr;
         o.bytes = 0;
         o.borrowed = false;
+        o.dev = -1;
     }
     template <typename T>
     T *as() const { return (T *)ptr; }
@@ -48,6 +58,7 @@ struct DevBuf {
         ptr = nullptr;
         bytes = 0;
         borrowed = false;
+        dev = -1;
         return p;
     }
 };
@@ -126,6 +137,13 @@ struct fsamd_plan {
     std::vector<void *> retired;
     ~fsamd_plan()
     {
+        // the buffers go back to the device pool once no queued work can use them
+        (void)hipDeviceSynchronize();
+        for (fsamd::DevBuf *b : {&axis, &cofm, &cell_start, &cell_ids, &cell_kl, &poff, &line_off, &pair_part,
+                                 &order, &seg, &queue, &eager_lines.buf, &stats, &rec, &fix, &fix_count, &win})
+            b->recycle();
+        for (auto &it : item_sets) it->buf.recycle();
+        for (auto &t : pinned_lines) t->buf.recycle();
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         for (void *p : retired)
