@@ -601,16 +601,18 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
 constexpr int kWaveQueue = kWaveClaim;
 
 struct PairLite {
-    double vel, inv_b, scale, vhigh, aa, inv_nm1;
+    double vel, inv_b, scale, vhigh, inv_nm1;
     long long k;      // pair index
     float W[4];       // kernel weights of the centre sample and the 3 above it
     float Cw[kWingTerms];   // far-wing coefficients (wing_coeffs)
+    VoigtA va;        // constants of the FP32 profile
     float ufar;
     int zmax;
     int shape;        // nm1 | rounds << 8 | pstep << 16 (step_shape)
     int pad;
 };
-static_assert(sizeof(PairLite) == 128, "PairLite layout");
+static_assert(sizeof(PairLite) == 160, "PairLite layout");
+static_assert(sizeof(VoigtA) == kWingTerms * sizeof(float), "the activation copies VoigtA with the Cw lanes");
 
 // The far-wing coefficients of wing_setup for one pair, all ten on one lane
 // (the batched set-up of the main kernel), in FP32: every term is positive,
@@ -706,7 +708,7 @@ __device__ inline void refill_wave(const DepositArgs &A, const WorkItem &w, int 
         r.inv_b = C.inv_b;
         r.scale = C.scale;
         r.vhigh = C.vhigh;
-        r.aa = C.aa;
+        r.va = make_voigt_a(C.aa);
         r.inv_nm1 = C.np > 1 ? 1.0 / (C.np - 1) : 1.0;
         r.k = kk;
         const double inv_vs = 1.0 / C.vsmooth;
@@ -744,7 +746,10 @@ __device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane, d
         set_sample_offset(P.smp[glane], ((glane + 1) * dvk - r.vhigh) * r.inv_b);
         P.smp[glane].w = r.W[glane < 3 ? 3 - glane : glane - 3];   // K(q) is even in the sample offset
     }
-    if (glane < kWingTerms) P.Cw[glane] = r.Cw[glane];
+    if (glane < kWingTerms) {
+        P.Cw[glane] = r.Cw[glane];
+        reinterpret_cast<float *>(&P.va)[glane] = reinterpret_cast<const float *>(&r.va)[glane];
+    }
     if (glane == 0) {
         const int nm1 = r.shape & 0xff;
         P.vel = r.vel;
@@ -752,10 +757,8 @@ __device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane, d
         if (kStats) P.b = 1.0 / r.inv_b;
         P.dvk = dvk;
         P.vhigh = r.vhigh;
-        P.aa = r.aa;
         P.zmax = r.zmax;
         P.fp64 = 0;
-        P.va = make_voigt_a(r.aa);
         P.scale = r.scale;
         P.np = nm1 + 1;
         P.ufar = (double)r.ufar;
