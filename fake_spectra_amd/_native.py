@@ -28,7 +28,8 @@ EXPORTS = ("fsamd_last_error", "fsamd_version", "fsamd_plan_create", "fsamd_plan
            "fsamd_tau_autotune", "fsamd_plan_overflow", "fsamd_mean_flux_scale", "fsamd_res_corr",
            "fsamd_flux_power", "fsamd_flux_hist", "fsamd_last_kernel_ms", "fsamd_set_block_waves",
            "fsamd_voigt_eval", "fsamd_plan_window_generation", "fsamd_set_fp64",
-           "fsamd_particle_interpolate_noalloc", "fsamd_near_lines", "fsamd_plan_assign_cells")
+           "fsamd_particle_interpolate_noalloc", "fsamd_near_lines", "fsamd_plan_assign_cells",
+           "fsamd_pool_release")
 
 
 class FsamdLine(C.Structure):
@@ -97,6 +98,8 @@ def _declare(L):
     L.fsamd_plan_window_generation.restype = i
     L.fsamd_set_fp64.argtypes = [i]
     L.fsamd_set_fp64.restype = i
+    L.fsamd_pool_release.argtypes = []
+    L.fsamd_pool_release.restype = i
     L.fsamd_tau_adjoint.argtypes = [vp, i, i, d, vp, vp, vp, vp, C.POINTER(FsamdLineGrad), i, vp, vp, vp]
     L.fsamd_tau_adjoint.restype = i
     L.fsamd_tau_autotune.argtypes = [vp, i, i, d, d, vp, vp, vp, vp, C.POINTER(FsamdLine), i, vp, C.POINTER(i),

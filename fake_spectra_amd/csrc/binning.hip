@@ -863,6 +863,19 @@ void pool_flush(int dev)
 }
 }  // namespace
 
+}  // namespace fsamd
+
+extern "C" int fsamd_pool_release(void)
+{
+    int d = 0;
+    FSAMD_HIP(hipGetDevice(&d));
+    FSAMD_HIP(hipDeviceSynchronize());
+    fsamd::pool_flush(d);
+    return FSAMD_OK;
+}
+
+namespace fsamd {
+
 DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release()

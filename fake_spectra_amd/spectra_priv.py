@@ -337,6 +337,12 @@ def set_fp64(on: bool):
     N.check(N.lib().fsamd_set_fp64(1 if on else 0))
 
 
+def release_pool():
+    """Free the device buffers the library keeps from destroyed plans on the
+    current device (fsamd_pool_release)."""
+    N.check(N.lib().fsamd_pool_release())
+
+
 def set_stats(enable: bool, timing_only: bool = False):
     """Collect the counters and GPU time of each tau call (last_stats), or
     with timing_only just the time (the kernels then run without counting)."""

@@ -219,6 +219,14 @@ int fsamd_plan_overflow(fsamd_plan *plan, int *flag, void *stream);
  * rounding.  Much slower; default off. */
 int fsamd_set_fp64(int on);
 
+/* Memory management with no counterpart in the reference: the device
+ * buffers of destroyed plans (and the binning's temporaries) stay in a pool
+ * of the device they were allocated on, up to 64 GiB per device, and later
+ * plans reuse them.  This frees the calling device's pooled buffers (after
+ * waiting for the device); the pool also empties itself when an allocation
+ * fails. */
+int fsamd_pool_release(void);
+
 /* Tuning knob with no counterpart in the reference: pixels walked per lane
  * group of the tau kernel (16, 32 or 64; a 64-lane wavefront walks 64/g
  * (pair, direction) streams at once).  Process-wide; default 16. */
