@@ -13,7 +13,8 @@ the tophat (0), cubic (1) and quintic (3) kernels:
   derivatives against FP64 arithmetic;
 * against central finite differences at the full workload (128^3 particles,
   1024 sightlines x 1024 pixels) on 64 random particles near the sightlines,
-  for density, temperature, line-of-sight velocity and neutral fraction.
+  for density, temperature, line-of-sight velocity and neutral fraction; the
+  same for the Voronoi kernel (2) on a 64^3-particle field.
   The difference quotients are taken with the FP64 forward (fsamd_set_fp64)
   on the sightlines the particle reaches, so they carry no FP32 rounding
   noise.  The forward is piecewise smooth: a perturbation can move a pixel
@@ -161,8 +162,25 @@ def test_adjoint_central_differences_full_workload(kernel):
     w = synth.config(3)
     assert w.npart == 128 ** 3 and w.nlos == 1024 and w.nbins == 1024
     w.kernel = kernel
+    _central_differences(w, seed=7 + kernel)
+
+
+def test_voronoi_adjoint_central_differences():
+    """Kernel 2 (each particle deposited over the segment of the sightline it
+    owns, IndexTable::assign_cells): the adjoint against central differences
+    on a 64^3-particle field with 64 sightlines, segments assigned once per
+    plan (they depend on positions only)."""
+    w = synth.config(3, nlos=64, n_side=64)
+    w.kernel = 2
+    _central_differences(w, seed=11, assign=True)
+
+
+def _central_differences(w, seed, assign=False):
+    kernel = w.kernel
     pos, hh, axis, cofm = dev(w.pos), dev(w.hh), dev(w.axis), dev(w.cofm)
     plan = SP.Plan(w.box, pos, hh, axis, cofm)
+    if assign:
+        plan.assign_cells()
     F = _target(w, plan)
     xh = dev(w.neutral_fraction)
     nfac_h = (w.dens["H1"] / w.neutral_fraction).astype(np.float32)
@@ -179,7 +197,7 @@ def test_adjoint_central_differences_full_workload(kernel):
              "vel": v.grad.double().cpu().numpy()[:, 0], "xhi": x.grad.double().cpu().numpy()}
     los, parts = _rows_of(plan, w.nlos)
     del tau
-    rng = np.random.default_rng(7 + kernel)
+    rng = np.random.default_rng(seed)
     sel = rng.choice(np.unique(parts), size=64, replace=False)
     base = {"dens": (x * nfac).detach(), "temp": dtemp, "vel": dvel, "xhi": xh}
     fd = {k: [] for k in grads}
@@ -189,6 +207,8 @@ def test_adjoint_central_differences_full_workload(kernel):
             rows = np.unique(los[parts == p])
             sub = SP.Plan(w.box, pos, hh, axis[torch.as_tensor(rows, device="cuda")],
                           cofm[torch.as_tensor(rows, device="cuda")])
+            if assign:
+                sub.assign_cells()
             Fs = F[torch.as_tensor(rows, device="cuda")]
 
             def loss_of(name, val):
