@@ -195,31 +195,38 @@ __device__ inline float sample_u(const KernelSample &k, Split s)
     return (k.cv_hi - s.hi) + (k.cv_lo - s.lo);
 }
 
-struct PairTau {
-    KernelSample smp[kNGrid - 1];   // kernel sample velocities / b and weights
+// The constants a walk of the main kernel reads, in one layout in the pair
+// set-up record and in the group's PairTau slot, so that taking a record
+// copies them with one 8-byte move per lane.
+struct WalkHead {
     double vel;      // velfac*ppos + pvel (absorption.cpp:234)
-    double b;        // btherm (absorption.cpp:220)
     double inv_b;
-    double dvk;      // 2 vhigh / NGRID (singleabs.h:146)
-    double vhigh, vdr2, vsmooth, aa;
     double scale;    // amp * dens * dvk / velfac: taulast = scale * (sample sum)
+    double inv_nm1;  // 1 / (np - 1)
     // Far wing: for a pixel centred at U (units of b) with |U| >= ufar the
     // 7 x np samples sum to Y (C0 + C1 Y + ... + C9 Y^9), Y = (ufar/U)^2.
     double ufar;
-    float ufar2;
+    int zmax;
+    int nm1, rounds, pstep;   // near-step shape of the main kernel (step_shape)
+};
+static_assert(sizeof(WalkHead) == 56, "WalkHead is seven 8-byte words");
+
+struct PairTau : WalkHead {
+    KernelSample smp[kNGrid - 1];   // kernel sample velocities / b and weights
+    double b;        // btherm (absorption.cpp:220)
+    double dvk;      // 2 vhigh / NGRID (singleabs.h:146)
+    double vhigh, vdr2, vsmooth, aa;
+    float ufar2;     // ufar^2 (fix-up and adjoint kernels)
     double scale_nd; // amp * dvk / velfac (scale without the density)
     double temp;     // particle temperature
     int part;        // particle index
     float Cw[kWingTerms];
     float CwF[kWingTerms];   // adjoint: far-wing coefficients of sum w (u dH/du + a dH/da)
     VoigtA va;
-    int zmax;
     int fp64;
     int np;          // sub-samples per pixel (1 = midpoint)
     int wing_ready;  // Cw computed
     double dv;       // sub-sample spacing bintov / (np - 1)
-    double inv_nm1;  // 1 / (np - 1)
-    int nm1, rounds, pstep;   // near-step shape of the main kernel (step_shape)
 };
 
 __device__ inline void wave_sync()
@@ -239,7 +246,9 @@ __device__ inline double shfl_d(double v, int src)
 }
 
 // sum_i K(q_i) Re w((vv_i - vouter)/b + i a), FP32 profile; the sample
-// offsets from the FP64 anchor vouter/b (one split per sum)
+// offsets from the FP64 anchor vouter/b (one split per sum).  kA3 = false:
+// without the profile's a^3 term (pairs with a <= kA3Drop).
+template <bool kA3 = true>
 __device__ inline double inner32(const PairTau &P, double vouter, const float *gtab)
 {
     const Split s = split_s(vouter * P.inv_b);
@@ -248,7 +257,7 @@ __device__ inline double inner32(const PairTau &P, double vouter, const float *g
 #pragma unroll 2  // measured 0.7% faster at c1 than 1; a full unroll spills
     for (int i = 0; i < kNGrid - 1; ++i) {
         const KernelSample k = P.smp[i];
-        acc = __builtin_fmaf(k.w, voigt32(va, sample_u(k, s), gtab), acc);
+        acc = __builtin_fmaf(k.w, voigt32<kA3>(va, sample_u(k, s), gtab), acc);
     }
     return (double)acc;
 }
@@ -601,17 +610,14 @@ __device__ inline bool setup_pair(const DepositArgs &A, int line, const LineDesc
 constexpr int kWaveQueue = kWaveClaim;
 
 struct PairLite {
-    double vel, inv_b, scale, vhigh, inv_nm1;
+    WalkHead h;
+    double vhigh;
     long long k;      // pair index
     float W[4];       // kernel weights of the centre sample and the 3 above it
     float Cw[kWingTerms];   // far-wing coefficients (wing_coeffs)
     VoigtA va;        // constants of the FP32 profile
-    float ufar;
-    int zmax;
-    int shape;        // nm1 | rounds << 8 | pstep << 16 (step_shape)
-    int pad;
 };
-static_assert(sizeof(PairLite) == 160, "PairLite layout");
+static_assert(sizeof(PairLite) == 168, "PairLite layout");
 static_assert(sizeof(VoigtA) == kWingTerms * sizeof(float), "the activation copies VoigtA with the Cw lanes");
 
 // The far-wing coefficients of wing_setup for one pair, all ten on one lane
@@ -704,12 +710,16 @@ __device__ inline void refill_wave(const DepositArgs &A, const WorkItem &w, int 
     wave_sync();   // the records taken earlier have been read
     if (enq) {
         PairLite &r = q[(tail + __popcll(m & ((1ull << lane) - 1ull))) % kWaveQueue];
-        r.vel = C.vel;
-        r.inv_b = C.inv_b;
-        r.scale = C.scale;
+        r.h.vel = C.vel;
+        r.h.inv_b = C.inv_b;
+        r.h.scale = C.scale;
         r.vhigh = C.vhigh;
         r.va = make_voigt_a(C.aa);
-        r.inv_nm1 = C.np > 1 ? 1.0 / (C.np - 1) : 1.0;
+        // the walk leaves out the profile's a^3 term for a <= kA3Drop, as a
+        // property of the pair (the result does not depend on which pairs
+        // share a wavefront)
+        if (C.aa <= (double)kA3Drop) r.va.a3_sqpi3 = 0.0f;
+        r.h.inv_nm1 = C.np > 1 ? 1.0 / (C.np - 1) : 1.0;
         r.k = kk;
         const double inv_vs = 1.0 / C.vsmooth;
         float W[4];
@@ -719,13 +729,15 @@ __device__ inline void refill_wave(const DepositArgs &A, const WorkItem &w, int 
             W[i] = (float)kernel_weight(A.kernel, sqrt(C.vdr2 + vv * vv) * inv_vs);
             r.W[i] = W[i];
         }
+        // the walk compares with ufar rounded to FP32, the unit of the coefficients
         const float ufar = (float)C.ufar;
-        r.ufar = ufar;
-        r.zmax = C.zmax;
+        r.h.ufar = (double)ufar;
+        r.h.zmax = C.zmax;
         const StepShape sh = step_shape(C.np, kG);
-        r.shape = sh.nm1 | (sh.rounds << 8) | (sh.pstep << 16);
-        r.pad = 0;
-        // far-wing coefficients, in units of the ufar the walk compares with
+        r.h.nm1 = sh.nm1;
+        r.h.rounds = sh.rounds;
+        r.h.pstep = sh.pstep;
+        // far-wing coefficients, in units of that ufar
         const double iu = 1.0 / (double)ufar;
         wing_coeffs((float)(C.vhigh * C.inv_b * iu), (float)(0.5 * bintov * C.inv_b * iu), sh.nm1, W[0], W[1], W[2],
                     W[3], (float)C.aa, ufar, r.Cw);
@@ -735,43 +747,31 @@ __device__ inline void refill_wave(const DepositArgs &A, const WorkItem &w, int 
     more = (long long)base + want < w.p1;
 }
 
-// Expand queued record r into the group's PairTau slot (the fields the main
-// kernel's walk reads; setup_pair and wing_setup fill the same values).
-// Only the lanes of the groups taking a record call it.
+// Expand queued record r into the group's PairTau slot: the fields the main
+// kernel's walk reads, one 8-byte word, weight, coefficient or sample offset
+// per lane (setup_pair and wing_setup fill the same values for the other
+// kernels).  Only the lanes of the groups taking a record call it.
 template <bool kStats>
-__device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane, double bintov)
+__device__ inline void activate_pair(const PairLite &r, PairTau &P, int glane)
 {
-    const double dvk = 2. * r.vhigh / kNGrid;
     if (glane < kNGrid - 1) {
-        set_sample_offset(P.smp[glane], ((glane + 1) * dvk - r.vhigh) * r.inv_b);
+        const double dvk = 2. * r.vhigh / kNGrid;
+        set_sample_offset(P.smp[glane], ((glane + 1) * dvk - r.vhigh) * r.h.inv_b);
         P.smp[glane].w = r.W[glane < 3 ? 3 - glane : glane - 3];   // K(q) is even in the sample offset
     }
     if (glane < kWingTerms) {
         P.Cw[glane] = r.Cw[glane];
         reinterpret_cast<float *>(&P.va)[glane] = reinterpret_cast<const float *>(&r.va)[glane];
     }
-    if (glane == 0) {
-        const int nm1 = r.shape & 0xff;
-        P.vel = r.vel;
-        P.inv_b = r.inv_b;
-        if (kStats) P.b = 1.0 / r.inv_b;
-        P.dvk = dvk;
-        P.vhigh = r.vhigh;
-        P.zmax = r.zmax;
-        P.fp64 = 0;
-        P.scale = r.scale;
-        P.np = nm1 + 1;
-        P.ufar = (double)r.ufar;
-        P.ufar2 = (float)((double)r.ufar * (double)r.ufar);
-        P.wing_ready = 1;
-        P.inv_nm1 = r.inv_nm1;
-        P.dv = nm1 > 0 ? bintov * r.inv_nm1 : 0.0;
-        P.nm1 = nm1;
-        P.rounds = (r.shape >> 8) & 0xff;
-        P.pstep = r.shape >> 16;
+    constexpr int kHeadWords = sizeof(WalkHead) / sizeof(long long);
+    if (glane < kHeadWords)
+        reinterpret_cast<long long *>(static_cast<WalkHead *>(&P))[glane] =
+            reinterpret_cast<const long long *>(&r.h)[glane];
+    if (kStats && glane == kHeadWords) {
+        // read by the counters (pixel_samples)
+        P.b = 1.0 / r.h.inv_b;
     }
 }
-
 
 // Voigt samples the reference evaluates for one pixel (singleabs.h:110-125).
 __device__ inline int pixel_samples(const PairTau &P, double vlow, double bintov)
@@ -939,7 +939,7 @@ __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *
             double t = 0;
             if (valid) {
                 const double U = (vlow + 0.5 * bintov) * P.inv_b;
-                const float Y = P.ufar2 * __builtin_amdgcn_rcpf((float)(U * U));
+                const float Y = (float)(P.ufar * P.ufar) * __builtin_amdgcn_rcpf((float)(U * U));
                 float S = P.Cw[kWingTerms - 1];
 #pragma unroll
                 for (int q = kWingTerms - 2; q >= 0; --q) S = __builtin_fmaf(S, Y, P.Cw[q]);
@@ -958,6 +958,10 @@ __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *
     }
     const int nm1 = P.nm1;
     const int pcount = min(P.pstep, count - c);
+    // pairs with a <= kA3Drop carry no a^3 term (a3_sqpi3 = 0); when no pair
+    // of this near step of the wavefront has one, its code is skipped (a
+    // wavefront-uniform branch with the same results)
+    const bool a3 = __ballot(P.va.a3_sqpi3 != 0.0f) != 0ull;
     wave_sync();   // the previous step's reads of sbuf are done
     {
         const int nsub = nm1 == 0 ? pcount : pcount * nm1 + 1;
@@ -974,9 +978,9 @@ __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *
                     // sub-sample m of the step, in increasing velocity: sub-sample
                     // m mod (np-1) of its pixel (singleabs.h:118-121), up to the
                     // rounding of m dv against the pixel's own vlow
-                    v = __builtin_fma((double)m, P.dv, vlo);
+                    v = __builtin_fma((double)m, bintov * P.inv_nm1, vlo);
                 }
-                sbuf[m] = (float)inner32(P, v, gtab);
+                sbuf[m] = (float)(a3 ? inner32<true>(P, v, gtab) : inner32<false>(P, v, gtab));
                 if (kStats) st.sums++;
             }
         }
@@ -1137,7 +1141,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 if (dir == 0) {
                     if (rank < avail) {
                         const PairLite &r = q[(head + rank) % kWaveQueue];
-                        activate_pair<kStats>(r, P, glane, bintov);
+                        activate_pair<kStats>(r, P, glane);
                         k = r.k;
                         dir = 1;
                         c = 0;
