@@ -715,10 +715,15 @@ __device__ inline void refill_wave(const DepositArgs &A, const WorkItem &w, int 
         r.h.scale = C.scale;
         r.vhigh = C.vhigh;
         r.va = make_voigt_a(C.aa);
-        // the walk leaves out the profile's a^3 term for a <= kA3Drop, as a
-        // property of the pair (the result does not depend on which pairs
-        // share a wavefront)
-        if (C.aa <= (double)kA3Drop) r.va.a3_sqpi3 = 0.0f;
+        // the walk uses the light form of the profile for a <= kA3Drop, as a
+        // property of the pair: its coefficients are zero, so the full form
+        // gives the same values and the result does not depend on which
+        // pairs share a wavefront
+        if (C.aa <= (double)kA3Drop) {
+            r.va.a3_sqpi3 = 0.0f;
+            r.va.lc2 = 0.0f;
+            r.va.lc3 = 0.0f;
+        }
         r.h.inv_nm1 = C.np > 1 ? 1.0 / (C.np - 1) : 1.0;
         r.k = kk;
         const double inv_vs = 1.0 / C.vsmooth;
@@ -958,9 +963,10 @@ __device__ inline bool walk_step(const DepositArgs &A, const PairTau &P, float *
     }
     const int nm1 = P.nm1;
     const int pcount = min(P.pstep, count - c);
-    // pairs with a <= kA3Drop carry no a^3 term (a3_sqpi3 = 0); when no pair
-    // of this near step of the wavefront has one, its code is skipped (a
-    // wavefront-uniform branch with the same results)
+    // pairs with a <= kA3Drop carry the light form of the profile (zero a^3
+    // and exponent coefficients); when every pair of this near step of the
+    // wavefront does, the light code runs (a wavefront-uniform branch with
+    // the same results)
     const bool a3 = __ballot(P.va.a3_sqpi3 != 0.0f) != 0ull;
     wave_sync();   // the previous step's reads of sbuf are done
     {

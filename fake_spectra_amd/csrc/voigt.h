@@ -87,14 +87,17 @@ __device__ inline VoigtA make_voigt_a(double a)
     return v;
 }
 
-// Damping parameters up to which the deposit may drop the a^3 term: for
-// a <= kA3Drop it is at most 2.9e-7 of H over all u (at u ~ 3.3; measured
-// against scipy.special.wofz), below the expansion's own error.
-constexpr float kA3Drop = 1.5e-3f;
+// Damping parameters up to which the deposit uses the light form of the
+// profile: without the a^3 term and with the Gaussian exponent cut to
+// a^2 - (1 + 2a^2) u^2.  For a <= kA3Drop the two together change H by at
+// most 5.5e-7 relative over all u (the a^3 term at u ~ 3.2; measured against
+// scipy.special.wofz).
+constexpr float kA3Drop = 2e-3f;
 
 // H(a, u) with u the sample's offset from the line centre in units of b,
-// any u.  The table `tab` is 16-byte aligned.  kA3 = false leaves out the
-// a^3 term (for a <= kA3Drop).
+// any u.  The table `tab` is 16-byte aligned.  kA3 = false: the light form
+// (for a <= kA3Drop), which equals the full one when a3_sqpi3, lc2 and lc3
+// are zero.
 template <bool kA3 = true>
 __device__ inline float voigt32(const VoigtA &v, float u, const float *tab)
 {
@@ -108,7 +111,7 @@ __device__ inline float voigt32(const VoigtA &v, float u, const float *tab)
     // u^4 and u^6 terms are small next to the u^2 one, so y keeps the
     // rounding of its first fma.  2^y is 0 once y passes the FP32 range.
     const float y = __builtin_fmaf(p, v.lc1, v.a2_log2e);
-    const float re_exp = __builtin_amdgcn_exp2f(__builtin_fmaf(p * p, __builtin_fmaf(p, v.lc3, v.lc2), y));
+    const float re_exp = __builtin_amdgcn_exp2f(kA3 ? __builtin_fmaf(p * p, __builtin_fmaf(p, v.lc3, v.lc2), y) : y);
     // G: table piece (|u| < 7) or x R(x) (wing row), selected per lane
     const float x = __builtin_amdgcn_rcpf(p);
     const bool core = uh < kVoigtCoreMax;
