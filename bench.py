@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-near-lines", action="store_true")
     ap.add_argument("--no-drop-in", action="store_true")
+    ap.add_argument("--no-flux", action="store_true", help="skip the flux-statistics leg")
     ap.add_argument("--adjoint-config", type=int, default=3,
                     help="workload of the forward vs forward+adjoint leg (3: 128^3, 1024 x 1024, H I 1215)")
     ap.add_argument("--colden", action="store_true",
@@ -312,6 +313,40 @@ def colden_leg(w, pos, hh, ax, cofm, dens, steps):
             "note": "binning + batched colden launch"}
 
 
+def flux_leg(w, tau, reps=3):
+    """The flux statistics on the device (fsamd_mean_flux_scale,
+    fsamd_flux_power, fsamd_res_corr, fsamd_flux_hist) over one line's
+    optical depths as the deposit left them in HBM ([nlos, nbins] float64):
+    milliseconds per call (host round trip included, as the reference's
+    fluxstatistics.py functions return to the host) and the tau bytes read
+    per millisecond."""
+    import torch
+    from fake_spectra_amd import fluxstatistics as GF
+    vmax = w.box * w.velfac
+    dvbin = vmax / w.nbins
+    calls = {
+        "mean_flux": lambda: GF.mean_flux(tau, 0.7),
+        "flux_power": lambda: GF.flux_power(tau, vmax, mean_flux_desired=0.7),
+        "res_corr": lambda: GF.res_corr(tau, dvbin, 8.0),
+        "flux_pdf": lambda: GF.flux_pdf(tau, 20, 0.7),
+    }
+    nbytes = tau.numel() * 8
+    res = {"tau_bytes": nbytes, "rows": int(tau.shape[0]), "pixels": int(tau.shape[1])}
+    for name, f in calls.items():
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        res[name] = {"ms": ms, "tau_GB_per_s": nbytes / (ms * 1e-3) / 1e9}
+    res["note"] = ("mean_flux: Newton-Raphson, one reduction pass per iteration; flux_power and flux_pdf "
+                   "include their mean-flux rescale; flux_power is the mixed-radix FFT, two sightlines per "
+                   "complex transform")
+    return res
+
+
 def cyc_split(st):
     """Fractions of the counting kernel's wavefront cycles in pair set-up,
     far-wing set-up and walk steps, and the share of wave iterations that set
@@ -512,6 +547,9 @@ def main():
     if args.colden:
         out["colden"] = colden_leg(w, pos, hh, ax, cofm, dens[0], args.steps)
         progress("colden done")
+    if not args.no_flux:
+        out["flux"] = flux_leg(w, outs[0])
+        progress("flux statistics done")
     if not args.no_adjoint:
         same = args.adjoint_config == args.config
         out["adjoint"] = adjoint_leg(max(2, args.steps), args.adjoint_config, w if same else None)

@@ -709,6 +709,9 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         FSAMD_HIP(hipStreamSynchronize(s));
         if ((rc = regrow(P, P->rec, rbytes))) return rc;
     }
+    // the list of pairs for the FP64 adjoint kernel (the forward's fix-up list)
+    if (!P->fix_count.ptr && (rc = P->fix_count.alloc(sizeof(unsigned long long) * 2, P->arena))) return rc;
+    if (P->fix_cap < 65536 && (rc = grow_fixups(P, 65536, s))) return rc;
     const int gsize = g_group_size.load(std::memory_order_relaxed);
     for (int g = 0; g < ngroups; ++g) {
         const int k0 = g * kMaxLines;
@@ -744,9 +747,22 @@ int fsamd_tau_adjoint(fsamd_plan *P, int nbins, int kernel, double velfac, const
         if ((rc = item_set(P, K, 1ll << 62, &set))) return rc;
         if (set->host.empty()) continue;
         A.nitems = (int)set->host.size();
+        A.fix = P->fix.as<Fixup>();
+        A.fix_count = P->fix_count.as<unsigned long long>();
+        A.fix_cap = P->fix_cap;
         if ((rc = launch_fill_i32(A.queue, 0, 1, s))) return rc;
+        if ((rc = launch_fill_i32((int *)A.fix_count, 0, 4, s))) return rc;
         const int nblocks = (int)std::min<long long>((long long)adjoint_blocks_per_cu(gsize, nbins) * dev.ncu, A.nitems);
         if ((rc = launch_tau_adjoint(gsize, A, set->buf.as<WorkItem>(), nblocks, s))) return rc;
+        if ((rc = launch_tau_adjoint64(A, set->buf.as<WorkItem>(), dev.ncu * 2, s))) return rc;
+        unsigned long long fc[2] = {0, 0};
+        FSAMD_HIP(hipMemcpyAsync(fc, A.fix_count, sizeof(fc), hipMemcpyDeviceToHost, s));
+        FSAMD_HIP(hipStreamSynchronize(s));
+        // the forward listed every FP64 pair of the group in this list, so it
+        // holds them unless the FP64 setting changed in between
+        if (fc[1])
+            return fail(FSAMD_ERR_UNSUPPORTED, "adjoint: more FP64 pairs than the forward's fix-up list holds; "
+                                               "run the forward with the same fsamd_set_fp64 setting");
     }
     return FSAMD_OK;
 }

@@ -27,6 +27,8 @@ int launch_deposit(bool tau, bool lds, int gsize, int waves, const DepositArgs &
 int deposit_blocks_per_cu(bool tau, bool lds, int gsize, int waves, int nbins);
 int effective_waves(bool lds, bool stats, int waves);
 int launch_tau_adjoint(int gsize, const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);
+// the FP64 pairs tau_adjoint_kernel listed in A.fix (after it, on the same stream)
+int launch_tau_adjoint64(const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);
 int adjoint_blocks_per_cu(int gsize, int nbins);
 int launch_tau_fixup(const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s);
 int launch_voigt_eval(const double *u, long long n, double a, float *out, hipStream_t s);

@@ -1425,25 +1425,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             const int zlo = P.zmax - lb;   // the window: pixels zlo .. zlo + n - 1
             double aS = 0, aU = 0, aF = 0;
             if (P.fp64) {
-                // the forward walked this pair with the FP64 profile, pixel by pixel
-                for (int j = glane; j < n; j += kG) {
-                    const int z = zlo + j;
-                    double S, SU, SF;
-                    adj_pixel64(P, z * bintov - P.vel, bintov, A.kernel, S, SU, SF);   // absorption.cpp:252
-                    const double g = grow[wrap_pixel(z, A.nbins)];
-                    aS += g * S;
-                    aU += g * SU;
-                    aF += g * SF;
-                }
-                aS = group_sum<kG>(aS);
-                aU = group_sum<kG>(aU);
-                aF = group_sum<kG>(aF);
-                if (glane == 0 && n > 0) {
-                    const int p = P.part;
-                    unsafeAtomicAdd(&L.grad_dens[p], P.scale_nd * aS);
-                    unsafeAtomicAdd(&A.grad_vel[3 * (size_t)p + los.ax - 1], P.scale * P.inv_b * aU);
-                    unsafeAtomicAdd(&A.grad_temp[p], -P.scale / (2.0 * P.temp) * (aS + aF));
-                }
+                // the forward walked this pair with the FP64 profile, pixel by
+                // pixel: tau_adjoint64_kernel does the same (the FP64 Faddeeva
+                // code stays out of this kernel's registers)
+                if (glane == 0 && n > 0) push_fixup(A, k, item, 0, 0, 0);
                 continue;
             }
             // far-wing pixels (|U| >= ufar) sit at the two ends of the window
@@ -1519,6 +1504,58 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             }
         }
     }
+}
+
+// The adjoint of the pairs the forward evaluated in FP64 (listed by
+// tau_adjoint_kernel in the fix-up list): one kFixG-lane group per pair, the
+// window's pixels with the FP64 Faddeeva function and FP64 kernel weights,
+// dL/dtau read from HBM.
+__global__ __launch_bounds__(kThreads) void tau_adjoint64_kernel(DepositArgs A, const WorkItem *items)
+{
+    constexpr int kGroups = kThreads / kFixG;
+    __shared__ PairTau pslot[kGroups];
+    const int lane = threadIdx.x & 63, gid = threadIdx.x / kFixG, glane = lane % kFixG;
+    PairTau &P = pslot[gid];
+    const double bintov = A.vbox / A.nbins;
+    const int n = (int)min(A.fix_count[0], (unsigned long long)A.fix_cap);
+    for (int e = blockIdx.x * kGroups + gid; e < n; e += gridDim.x * kGroups) {
+        const Fixup f = A.fix[e];
+        const WorkItem w = items[f.item];
+        const LineDesc L = line_of(A, w.line);
+        const double *grad_row = L.out + (size_t)w.los * A.nbins;
+        const LosInfo los = los_info(A, w.los);
+        if (!setup_pair<kFixG>(A, w.line, L, los, f.k, P, glane)) continue;
+        const long long wi = ((long long)w.line * A.npairs + f.k) * 2;
+        const int lf = A.win[wi], lb = A.win[wi + 1];
+        const int nw = lf + lb;
+        const int zlo = P.zmax - lb;   // the window: pixels zlo .. zlo + nw - 1
+        double aS = 0, aU = 0, aF = 0;
+        for (int j = glane; j < nw; j += kFixG) {
+            const int z = zlo + j;
+            double S, SU, SF;
+            adj_pixel64(P, z * bintov - P.vel, bintov, A.kernel, S, SU, SF);   // absorption.cpp:252
+            const double g = grad_row[wrap_pixel(z, A.nbins)];
+            aS += g * S;
+            aU += g * SU;
+            aF += g * SF;
+        }
+        aS = group_sum<kFixG>(aS);
+        aU = group_sum<kFixG>(aU);
+        aF = group_sum<kFixG>(aF);
+        if (glane == 0) {
+            const int p = P.part;
+            unsafeAtomicAdd(&L.grad_dens[p], P.scale_nd * aS);
+            unsafeAtomicAdd(&A.grad_vel[3 * (size_t)p + los.ax - 1], P.scale * P.inv_b * aU);
+            unsafeAtomicAdd(&A.grad_temp[p], -P.scale / (2.0 * P.temp) * (aS + aF));
+        }
+    }
+}
+
+int launch_tau_adjoint64(const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s)
+{
+    tau_adjoint64_kernel<<<nblocks, kThreads, 0, s>>>(A, items);
+    FSAMD_HIP(hipGetLastError());
+    return FSAMD_OK;
 }
 
 int launch_tau_adjoint(int gsize, const DepositArgs &A, const WorkItem *items, int nblocks, hipStream_t s)
