@@ -280,18 +280,47 @@ __global__ void column_sum_kernel(const double *power, int nrow, int nk, double 
 // Histogram of exp(-s tau) over nb equal bins of [0, 1], bin edges e_k =
 // k / nb as numpy.histogram places them (searchsorted on the edges, the last
 // bin closed); values outside [0, 1] are not counted.
+__device__ inline int flux_bin(double f, int nb)
+{
+    int b = (int)(f * nb);
+    b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+    if (f < (double)b / (double)nb && b > 0) --b;
+    if (b < nb - 1 && f >= (double)(b + 1) / (double)nb) ++b;
+    return b;
+}
+
+// Counts are kept per block in LDS and added to the global bins once per
+// block: nb <= kFluxPrivBins bins get one private row per thread (no atomics;
+// a flux field piles up in few bins, which would serialize atomics), larger
+// nb one shared row with LDS atomics.
+constexpr int kFluxPrivBins = 32;
+
 __global__ __launch_bounds__(kFluxThreads) void flux_hist_kernel(const double *tau, long long n, double scale, int nb,
                                                                 unsigned long long *counts)
 {
+    extern __shared__ unsigned int hist[];   // [nb][kFluxThreads] private rows, or [nb] shared
+    const bool priv = nb <= kFluxPrivBins;
+    const int rows = priv ? nb * kFluxThreads : nb;
+    for (int i = threadIdx.x; i < rows; i += kFluxThreads) hist[i] = 0;
+    __syncthreads();
     const long long stride = (long long)gridDim.x * kFluxThreads;
     for (long long i = blockIdx.x * (long long)kFluxThreads + threadIdx.x; i < n; i += stride) {
         const double f = exp(-scale * tau[i]);
         if (!(f >= 0.0 && f <= 1.0)) continue;
-        int b = (int)(f * nb);
-        b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
-        if (f < (double)b / (double)nb && b > 0) --b;
-        if (b < nb - 1 && f >= (double)(b + 1) / (double)nb) ++b;
-        atomicAdd(&counts[b], 1ull);
+        const int b = flux_bin(f, nb);
+        if (priv)
+            ++hist[b * kFluxThreads + threadIdx.x];
+        else
+            atomicAdd(&hist[b], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kFluxThreads) {
+        unsigned long long c = 0;
+        if (priv)
+            for (int t = 0; t < kFluxThreads; ++t) c += hist[b * kFluxThreads + t];
+        else
+            c = hist[b];
+        if (c) atomicAdd(&counts[b], c);
     }
 }
 
@@ -437,7 +466,10 @@ int fsamd_flux_hist(const double *tau, long long n, double scale, int nbins, uns
     FSAMD_HIP(hipMemsetAsync(counts, 0, sizeof(unsigned long long) * nbins, s));
     if (n > 0) {
         const int nblocks = (int)std::min<long long>(4096, (n + kFluxThreads - 1) / kFluxThreads);
-        flux_hist_kernel<<<nblocks, kFluxThreads, 0, s>>>(tau, n, scale, nbins, counts);
+        const size_t shm = sizeof(unsigned int) * (size_t)(nbins <= kFluxPrivBins ? nbins * kFluxThreads : nbins);
+        if (shm > 160 * 1024) return fail(FSAMD_ERR_UNSUPPORTED, "flux histogram: too many bins for the LDS");
+        if (int rc = raise_lds_limit((const void *)flux_hist_kernel, shm)) return rc;
+        flux_hist_kernel<<<nblocks, kFluxThreads, shm, s>>>(tau, n, scale, nbins, counts);
         FSAMD_HIP(hipGetLastError());
     }
     FSAMD_HIP(hipStreamSynchronize(s));
