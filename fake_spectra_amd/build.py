@@ -37,8 +37,15 @@ def hipcc() -> str:
     return p
 
 
+# Per-source additions.  deposit.hip: LLVM's iterative occupancy-first
+# machine scheduler; the deposit kernels run at 4 waves per SIMD with every
+# vector register in use, and this schedule measured 1-2.5% faster on the
+# tau kernel (256^3 and 512^3 particle sets) than the default one.
+SOURCE_FLAGS = {"deposit.hip": ["-mllvm", "--amdgpu-sched-strategy=iterative-maxocc"]}
+
+
 def _compile(src: str, obj: str, extra=()):
-    cmd = [hipcc()] + HIPCC_FLAGS + list(extra) + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc()] + HIPCC_FLAGS + SOURCE_FLAGS.get(src, []) + list(extra) + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-8000:]))

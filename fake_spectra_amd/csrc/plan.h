@@ -137,7 +137,12 @@ struct fsamd_plan {
     std::vector<void *> retired;
     ~fsamd_plan()
     {
-        // the buffers go back to the device pool once no queued work can use them
+        // the buffers go back to their device's pool once no queued work can
+        // use them: wait for that device (which need not be the current one)
+        int cur = -1;
+        (void)hipGetDevice(&cur);
+        const int d = axis.dev >= 0 ? axis.dev : cur;
+        if (d != cur) (void)hipSetDevice(d);
         (void)hipDeviceSynchronize();
         for (fsamd::DevBuf *b : {&axis, &cofm, &cell_start, &cell_ids, &cell_kl, &poff, &line_off, &pair_part,
                                  &order, &seg, &queue, &eager_lines.buf, &stats, &rec, &fix, &fix_count, &win})
@@ -148,5 +153,6 @@ struct fsamd_plan {
             if (e) (void)hipEventDestroy(e);
         for (void *p : retired)
             if (p) (void)hipFree(p);
+        if (d != cur) (void)hipSetDevice(cur);
     }
 };
