@@ -12,9 +12,9 @@
 //  a^5 term is below 1e-7 relative for a <= 0.04.
 //  |u| < 7:  G from a piecewise polynomial table, Z = -D'''/3 with
 //            D''' = (4u^2 - 6) G + 2;
-//  |u| >= 7: G = x R(x), x = 1/u^2, R a polynomial (the last table row) that
-//            follows the asymptotic series w ~ (i/sqrt(pi)) sum_n (2n-1)!!/2^n
-//            z^-(2n+1); Z = x^2 Q(x), Q = sum C(2n+3,3) (2n-1)!!/2^n x^n (3 terms),
+//  |u| >= 7: G = x R(x), x = 1/u^2, R a polynomial that follows the
+//            asymptotic series w ~ (i/sqrt(pi)) sum_n (2n-1)!!/2^n z^-(2n+1)
+//            (the last table row holds x R(x) as a polynomial in x); Z = x^2 Q(x), Q = sum C(2n+3,3) (2n-1)!!/2^n x^n (3 terms),
 //            the a^3 term of the same series, free of the cancellation in D'''.
 //  Each lane picks its table row (a core piece or the wing row) and its a^3
 //  form by selects, and every lane runs the same instructions, so the
@@ -112,7 +112,8 @@ __device__ inline float voigt32(const VoigtA &v, float u, const float *tab)
     // rounding of its first fma.  2^y is 0 once y passes the FP32 range.
     const float y = __builtin_fmaf(p, v.lc1, v.a2_log2e);
     const float re_exp = __builtin_amdgcn_exp2f(kA3 ? __builtin_fmaf(p * p, __builtin_fmaf(p, v.lc3, v.lc2), y) : y);
-    // G: table piece (|u| < 7) or x R(x) (wing row), selected per lane
+    // G: table piece (|u| < 7) or the wing row, selected per lane; both rows
+    // are polynomials in their variable, so G is the row's Horner sum
     const float x = __builtin_amdgcn_rcpf(p);
     const bool core = uh < kVoigtCoreMax;
     const float q = uh * (1.0f / FSAMD_G_WIDTH);
@@ -128,8 +129,7 @@ __device__ inline float voigt32(const VoigtA &v, float u, const float *tab)
     r = __builtin_fmaf(r, var, lo.w);
     r = __builtin_fmaf(r, var, lo.z);
     r = __builtin_fmaf(r, var, lo.y);
-    r = __builtin_fmaf(r, var, lo.x);
-    const float G = core ? r : x * r;
+    const float G = __builtin_fmaf(r, var, lo.x);
     if constexpr (!kA3) return __builtin_fmaf(-v.two_a_sqpi, G, re_exp);
     // a^3 term, -(a^3/sqrt(pi)) Z = (a^3/(3 sqrt(pi))) D with D = D''' =
     // (4u^2 - 6) G + 2, or D = -3 Z = -3 x^2 (1 + 5x + 26.25x^2) in the wing
