@@ -101,7 +101,7 @@ constexpr float kA3Drop = 2e-3f;
 template <bool kA3 = true>
 __device__ inline float voigt32(const VoigtA &v, float u, const float *tab)
 {
-    static_assert(FSAMD_G_DEG == 6 && FSAMD_G_STRIDE == 8, "voigt32 evaluates 7 coefficients per row");
+    static_assert(FSAMD_G_DEG == 6 && FSAMD_G_STRIDE % 4 == 0 && FSAMD_G_STRIDE >= 8, "voigt32 evaluates 7 coefficients per row");
     const float uh = __builtin_fabsf(u);
     const float p = u * u;
     // Re exp(-(u + ia)^2) = exp(a^2 - u^2) cos(2au) = 2^y, with ln cos x =
